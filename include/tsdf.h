@@ -1,0 +1,174 @@
+/*
+ * tsdf.h -- C ABI of libtsdf_hip.so, the MI355X (gfx950) voxel-hashed TSDF
+ * engine: block allocation into a spatial hash, projective integrate /
+ * deintegrate, and the voxel-centre raycast.
+ *
+ * The ABI replaces the member functions of the reference class GPUVoxelGrid
+ * (src/VoxelGrid/GPUVoxelGrid.h:40-120).  Each entry point names the
+ * reference interface it stands in for.  Conventions:
+ *   - plain pointers and sizes only; `stream` is a hipStream_t passed as
+ *     void* (NULL = the null stream of the grid's device);
+ *   - image inputs are DEVICE pointers, row-major [H][W] uint16 depth in mm
+ *     (0 = no measurement) and optional [H][W][3] uint8 colour;
+ *   - poses are HOST float[16], row-major camera->world, rigid (the last
+ *     row must be 0 0 0 1, GPUVoxelGrid.cpp:52-57 / integrate.cpp:147-159);
+ *   - calls are asynchronous on `stream`; tsdf_sync() / tsdf_stats() wait and
+ *     report deferred device-side errors (pool overflow);
+ *   - the caller owns its inputs, the handle owns the hash table and the
+ *     brick pool; one handle is used by one thread at a time (the reference
+ *     serialises callers with tsdfLock, integrate.h:23);
+ *   - no exception or abort crosses the ABI: every failure is a status.
+ */
+#ifndef SUPERSCANNER_TSDF_H
+#define SUPERSCANNER_TSDF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tsdf_grid tsdf_grid;
+
+typedef enum {
+    TSDF_OK = 0,
+    TSDF_BAD_ARG = 1,        /* NULL handle/pointer, bad size or parameter          */
+    TSDF_BAD_DIMS = 2,       /* image shape differs from the grid's (H, W); the     */
+                             /* reference only prints here, GPUVoxelGrid.cpp:46-49   */
+    TSDF_MAP_FULL = 3,       /* brick pool or hash table overflowed (the reference  */
+                             /* leaves this unchecked, GPUVoxelGrid.cpp:497)        */
+    TSDF_HIP_ERROR = 4,      /* a HIP runtime call failed; see tsdf_last_error()    */
+    TSDF_NO_INTRINSICS = 5,  /* integrate/raycast before tsdf_set_intrinsics        */
+    TSDF_BAD_POSE = 6        /* non-finite or non-rigid pose                        */
+} tsdf_status;
+
+enum {
+    TSDF_STORAGE_F32 = 0,    /* fp32 sdf + fp32 weight, 8 B/voxel                    */
+    TSDF_STORAGE_F16 = 1     /* fp16 sdf + fp16 weight, 4 B/voxel, fp32 arithmetic  */
+};
+
+enum {
+    TSDF_RAY_UB_VOXEL_INDEX = 1, /* voxel index -1 or 8 from float rounding         */
+                                 /* (GPUVoxelGrid.h:192-221): reference reads out of */
+                                 /* the brick; here the pixel is zeroed and flagged  */
+    TSDF_RAY_UB_MISSING_OUT = 2  /* zero crossing whose outer voxel lies in a block  */
+                                 /* that is not in the map (GPUVoxelGrid.cpp:669-672)*/
+};
+
+typedef struct {
+    float voxel_length;     /* vl in metres (GPUVoxelGrid ctor `voxlength`)          */
+    float truncation;       /* d_trunc in metres (ctor `trunc`)                      */
+    int32_t height;         /* resi (ctor `resi_`)                                   */
+    int32_t width;          /* resj (ctor `resj_`)                                   */
+    uint32_t n_buckets;     /* hash slots, power of two                              */
+    uint32_t n_bricks;      /* brick pool capacity (reference: c / 512,              */
+                            /* GPUVoxelGrid.cpp:11,27)                               */
+    int32_t storage;        /* TSDF_STORAGE_*                                        */
+    int32_t with_color;     /* keep a running-average rgb per voxel                  */
+    float max_weight;       /* 0 = uncapped, as the reference (GPUVoxelGrid.h:62)    */
+    float min_depth;        /* rmin, 0.2 m (GPUVoxelGrid.h:56)                       */
+    float max_ray_length;   /* d_max, 10 m (GPUVoxelGrid.h:55)                       */
+    int32_t device;         /* HIP device ordinal                                    */
+} tsdf_params;
+
+typedef struct {
+    uint64_t n_bricks;      /* bricks allocated (voxelMap2.size())                   */
+    uint64_t capacity;      /* pool capacity (maxVoxelBlocks)                        */
+    uint64_t overflow;      /* allocations refused since the last reset              */
+    uint64_t last_touched;  /* bricks in the last integrate call's touched set       */
+    uint64_t total_touched; /* sum of touched-set sizes since reset                  */
+    uint64_t total_new;     /* bricks inserted since reset                           */
+    uint64_t total_updated; /* voxel updates since reset                             */
+    uint64_t total_valid_px;/* pixels with camera z > rmin since reset               */
+    uint64_t ray_hits;      /* raycast pixels with a zero crossing, since reset      */
+    uint64_t ray_ub;        /* raycast pixels flagged as reference UB, since reset   */
+    uint64_t ray_samples;   /* raycast voxel reads since reset                       */
+    uint64_t ray_probes;    /* raycast hash lookups since reset                      */
+} tsdf_stats_out;
+
+/* Fills the defaults of the reference: d_max 10 m, rmin 0.2 m, no weight
+ * cap, fp32 storage, no colour (GPUVoxelGrid.h:55-62). */
+void tsdf_default_params(tsdf_params *p);
+
+/* GPUVoxelGrid::GPUVoxelGrid(c, trunc, voxlength, resi, resj),
+ * GPUVoxelGrid.cpp:10-30.  Allocates the table and pool on p->device. */
+tsdf_status tsdf_create(const tsdf_params *p, tsdf_grid **out);
+
+/* Releases everything the handle owns (reference: destructor of the Views). */
+void tsdf_destroy(tsdf_grid *g);
+
+/* Empties the map (all bricks freed, counters zeroed); no reference
+ * counterpart beyond constructing a new GPUVoxelGrid. */
+tsdf_status tsdf_reset(tsdf_grid *g, void *stream);
+
+/* GPUVoxelGrid::setIntrinsic(space, PinholeCameraIntrinsic),
+ * GPUVoxelGrid.cpp:253-266: stores fx, fy and the reference's
+ * ci = H - cy, cj = cx convention. */
+tsdf_status tsdf_set_intrinsics(tsdf_grid *g, float fx, float fy, float cx, float cy);
+
+/* GPUVoxelGrid::integrate (sign = +1) and GPUVoxelGrid::deIntegrate
+ * (sign = -1), GPUVoxelGrid.cpp:41-83 / 95-136, the (space, rgb, depth,
+ * extr, withColor) overloads.  depth: device [H][pitch] uint16 mm; rgb:
+ * device [H][W][3] uint8 or NULL (colour is integrated only when the grid was
+ * created with with_color and rgb != NULL, the reference's withColor). */
+tsdf_status tsdf_integrate(tsdf_grid *g, const uint16_t *depth, int32_t pitch,
+                           const uint8_t *rgb, const float E_cam2world[16],
+                           int32_t sign, void *stream);
+
+/* Ordered batch of integrate (+1) / deintegrate (-1) operations applied in
+ * one pass over the affected bricks; bit-identical to calling
+ * tsdf_integrate for op 0, 1, ..., n_ops-1 in turn.  Replaces the serial
+ * loop of integrate.cpp:108-125 and the unused reintegratePP
+ * (postprocessing.cpp:31-113).  depths/rgbs: HOST arrays of n_ops DEVICE
+ * pointers (rgbs may be NULL); E: n_ops*16 host floats; signs: n_ops. */
+tsdf_status tsdf_integrate_batch(tsdf_grid *g, int32_t n_ops,
+                                 const uint16_t *const *depths, int32_t pitch,
+                                 const uint8_t *const *rgbs, const float *E,
+                                 const int32_t *signs, void *stream);
+
+/* Reintegration after a pose-graph update: for keyframe k = 0..n-1,
+ * deintegrate with E_old[k] then integrate with E_new[k] (the order of
+ * integrate.cpp:108-125), as one batch. */
+tsdf_status tsdf_reintegrate_batch(tsdf_grid *g, int32_t n, const uint16_t *const *depths,
+                                   int32_t pitch, const uint8_t *const *rgbs,
+                                   const float *E_old, const float *E_new, void *stream);
+
+/* GPUVoxelGrid::raycast(space, extr), GPUVoxelGrid.cpp:139-147 with
+ * raycastingTag :561-715.  points: device [H][W][3] float, world space,
+ * (0,0,0) on a miss (rayPcd).  colors: device [H][W][3] float in [0,1] or
+ * NULL (rayColorPcd).  depth: device [H][W] float camera-frame z of the
+ * point or NULL (extension).  ub_flags: device [H][W] uint8 or NULL. */
+tsdf_status tsdf_raycast(tsdf_grid *g, const float E_cam2world[16], float *points,
+                         float *colors, float *depth, uint8_t *ub_flags, void *stream);
+
+/* Waits for `stream`, then reports TSDF_MAP_FULL if any allocation was
+ * refused since the last reset. */
+tsdf_status tsdf_sync(tsdf_grid *g, void *stream);
+
+/* Waits for `stream` and reads the counters (voxelMap2.size(),
+ * maxVoxelBlocks and the per-call counters). */
+tsdf_status tsdf_stats(tsdf_grid *g, tsdf_stats_out *out, void *stream);
+
+/* Copies the allocated bricks into caller-owned DEVICE buffers, converted to
+ * fp32: coords [cap][3] int32 block indices, sdf/w [cap][512] (voxel
+ * x + 8y + 64z, GPUVoxelGrid.h:28-31), rgb [cap][512][3] or NULL.  Writes
+ * the brick count to *n_out (host) and copies min(n, cap) bricks.  The
+ * reference equivalent is the deep_copy of voxelMap2 in marchingCubesHost,
+ * GPUVoxelGrid.cpp:204-245. */
+tsdf_status tsdf_export_bricks(tsdf_grid *g, int32_t *coords, float *sdf, float *w,
+                               float *rgb, uint64_t cap, uint64_t *n_out, void *stream);
+
+/* Host helper: the fp32 pose data every kernel uses -- E^-1 of a rigid pose
+ * evaluated in double and rounded once (stand-in for Eigen's float inverse,
+ * GPUVoxelGrid.cpp:55) and Ez = normalize(third column) (:52-53). */
+tsdf_status tsdf_pose_prepare(const float E_cam2world[16], float E_inv[16], float Ez[3]);
+
+const char *tsdf_status_string(tsdf_status s);
+
+/* Text of the last error on this thread ("" if none). */
+const char *tsdf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

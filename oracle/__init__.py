@@ -1,0 +1,208 @@
+"""Python binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product never imports it.
+
+The C restatement (tsdf_oracle.c) follows src/VoxelGrid/GPUVoxelGrid.{h,cpp}
+of the reference line by line; see the citations there.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+UB_VOXEL_INDEX = 1
+UB_MISSING_OUT = 2
+
+
+class OrcCounts(C.Structure):
+    _fields_ = [("n_touched", C.c_int64), ("n_new", C.c_int64), ("n_upd", C.c_int64),
+                ("n_valid", C.c_int64)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class OrcRayCounts(C.Structure):
+    _fields_ = [("n_hit", C.c_int64), ("n_ub", C.c_int64), ("samples", C.c_int64),
+                ("probes", C.c_int64), ("u_vox", C.c_int64), ("u_blk", C.c_int64)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def build() -> None:
+    import subprocess
+    subprocess.run(["make", "-B", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    fp, ip, vp = P(C.c_float), P(C.c_int), C.c_void_p
+    L.orc_prime_hash.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.orc_prime_hash.restype = C.c_uint64
+    L.orc_f2i.argtypes = [C.c_float]
+    L.orc_f2i.restype = C.c_int
+    L.orc_vb_index.argtypes = [C.c_float, C.c_float]
+    L.orc_vb_index.restype = C.c_int
+    L.orc_vin_vb.argtypes = [fp, ip, C.c_float, C.c_float, ip]
+    L.orc_vin_vb.restype = None
+    L.orc_pose_prepare.argtypes = [fp, fp, fp]
+    L.orc_pose_prepare.restype = None
+    L.orc_pose_is_rigid.argtypes = [fp]
+    L.orc_pose_is_rigid.restype = C.c_int
+    L.orc_create.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, C.c_int64, C.c_int,
+                             C.c_float, C.c_float, C.c_float]
+    L.orc_create.restype = vp
+    L.orc_destroy.argtypes = [vp]
+    L.orc_destroy.restype = None
+    L.orc_set_intrinsics.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_set_intrinsics.restype = None
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = None
+    L.orc_integrate.argtypes = [vp, vp, C.c_int, vp, fp, C.c_int, P(OrcCounts)]
+    L.orc_integrate.restype = C.c_int
+    L.orc_raycast.argtypes = [vp, fp, vp, vp, vp, vp, C.c_int, P(OrcRayCounts)]
+    L.orc_raycast.restype = None
+    L.orc_n_bricks.argtypes = [vp]
+    L.orc_n_bricks.restype = C.c_int64
+    L.orc_export.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.orc_export.restype = None
+    L.orc_set_mutation.argtypes = [C.c_int]
+    L.orc_set_mutation.restype = None
+    _lib = L
+    return L
+
+
+def _f32(a, n):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+    assert a.size == n
+    return a
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def prime_hash(x: int, y: int, z: int) -> int:
+    return int(lib().orc_prime_hash(x, y, z))
+
+
+def f2i(f: float) -> int:
+    return int(lib().orc_f2i(f))
+
+
+def vb_index(a: float, vbl: float) -> int:
+    return int(lib().orc_vb_index(a, vbl))
+
+
+def vin_vb(p, vb, vbl: float, vl: float):
+    pa = _f32(p, 3)
+    ba = np.ascontiguousarray(np.asarray(vb, dtype=np.int32))
+    out = np.zeros(3, dtype=np.int32)
+    lib().orc_vin_vb(_fptr(pa), ba.ctypes.data_as(C.POINTER(C.c_int)), vbl, vl,
+                     out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out
+
+
+def pose_prepare(E):
+    Ea = _f32(E, 16)
+    Ei = np.zeros(16, np.float32)
+    Ez = np.zeros(3, np.float32)
+    lib().orc_pose_prepare(_fptr(Ea), _fptr(Ei), _fptr(Ez))
+    return Ei.reshape(4, 4), Ez
+
+
+def pose_is_rigid(E) -> bool:
+    return bool(lib().orc_pose_is_rigid(_fptr(_f32(E, 16))))
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def set_mutation(m: int) -> None:
+    """Plants a deliberate mistake in the oracle (0 = none); used only by the
+    tests that show the closed-form checks catch such mistakes."""
+    lib().orc_set_mutation(int(m))
+
+
+class OracleGrid:
+    """CPU restatement of GPUVoxelGrid (GPUVoxelGrid.h:40-120)."""
+
+    def __init__(self, voxel_length, truncation, height, width, max_bricks=1 << 22,
+                 with_color=False, max_weight=0.0, min_depth=0.2, max_ray_length=10.0):
+        self.H, self.W = int(height), int(width)
+        self.with_color = bool(with_color)
+        self.vl = float(voxel_length)
+        self._h = lib().orc_create(voxel_length, truncation, self.H, self.W, int(max_bricks),
+                                   int(with_color), max_weight, min_depth, max_ray_length)
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_intrinsics(self, fx, fy, cx, cy):
+        lib().orc_set_intrinsics(self._h, fx, fy, cx, cy)
+
+    def integrate(self, depth, E, sign=1, rgb=None):
+        d = np.ascontiguousarray(depth, dtype=np.uint16)
+        assert d.shape == (self.H, self.W)
+        c = None
+        if rgb is not None:
+            c = np.ascontiguousarray(rgb, dtype=np.uint8)
+            assert c.shape == (self.H, self.W, 3)
+        cnt = OrcCounts()
+        st = lib().orc_integrate(self._h, d.ctypes.data, self.W,
+                                 c.ctypes.data if c is not None else None,
+                                 _fptr(_f32(E, 16)), int(sign), C.byref(cnt))
+        return st, cnt.as_dict()
+
+    def deintegrate(self, depth, E, rgb=None):
+        return self.integrate(depth, E, -1, rgb)
+
+    def raycast(self, E, stats=False):
+        H, W = self.H, self.W
+        pts = np.zeros((H, W, 3), np.float32)
+        cols = np.zeros((H, W, 3), np.float32)
+        dep = np.zeros((H, W), np.float32)
+        ub = np.zeros((H, W), np.uint8)
+        cnt = OrcRayCounts()
+        lib().orc_raycast(self._h, _fptr(_f32(E, 16)), pts.ctypes.data, cols.ctypes.data,
+                          dep.ctypes.data, ub.ctypes.data, int(bool(stats)), C.byref(cnt))
+        return {"points": pts, "colors": cols, "depth": dep, "ub": ub, "counts": cnt.as_dict()}
+
+    def n_bricks(self) -> int:
+        return int(lib().orc_n_bricks(self._h))
+
+    def export(self):
+        n = self.n_bricks()
+        coords = np.zeros((n, 3), np.int32)
+        keys = np.zeros(n, np.int64)
+        sdf = np.zeros((n, 512), np.float32)
+        w = np.zeros((n, 512), np.float32)
+        rgb = np.zeros((n, 512, 3), np.float32) if self.with_color else None
+        lib().orc_export(self._h, coords.ctypes.data, keys.ctypes.data, sdf.ctypes.data,
+                         w.ctypes.data, rgb.ctypes.data if rgb is not None else None)
+        return {"coords": coords, "keys": keys, "sdf": sdf, "w": w, "rgb": rgb}

@@ -1,0 +1,118 @@
+"""ctypes binding of libtsdf_hip.so (the C ABI declared in include/tsdf.h).
+
+The library is built in-tree by ``make -C superscanner_amd``.  There is no
+fallback: if the shared object is missing, loading raises instead of running
+anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtsdf_hip.so")
+
+TSDF_OK = 0
+TSDF_BAD_ARG = 1
+TSDF_BAD_DIMS = 2
+TSDF_MAP_FULL = 3
+TSDF_HIP_ERROR = 4
+TSDF_NO_INTRINSICS = 5
+TSDF_BAD_POSE = 6
+
+STORAGE_F32 = 0
+STORAGE_F16 = 1
+
+RAY_UB_VOXEL_INDEX = 1
+RAY_UB_MISSING_OUT = 2
+
+# Every symbol include/tsdf.h declares (checked by tests/test_capi_exports.py).
+EXPORTS = (
+    "tsdf_default_params", "tsdf_create", "tsdf_destroy", "tsdf_reset",
+    "tsdf_set_intrinsics", "tsdf_integrate", "tsdf_integrate_batch",
+    "tsdf_reintegrate_batch", "tsdf_raycast", "tsdf_sync", "tsdf_stats",
+    "tsdf_export_bricks", "tsdf_pose_prepare", "tsdf_status_string",
+    "tsdf_last_error",
+)
+
+
+class TsdfParams(C.Structure):
+    _fields_ = [
+        ("voxel_length", C.c_float),
+        ("truncation", C.c_float),
+        ("height", C.c_int32),
+        ("width", C.c_int32),
+        ("n_buckets", C.c_uint32),
+        ("n_bricks", C.c_uint32),
+        ("storage", C.c_int32),
+        ("with_color", C.c_int32),
+        ("max_weight", C.c_float),
+        ("min_depth", C.c_float),
+        ("max_ray_length", C.c_float),
+        ("device", C.c_int32),
+    ]
+
+
+class TsdfStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "n_bricks", "capacity", "overflow", "last_touched", "total_touched",
+        "total_new", "total_updated", "total_valid_px", "ray_hits", "ray_ub",
+        "ray_samples", "ray_probes")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class TsdfError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str):
+        self.status = status
+        super().__init__(f"{where}: status {status}: {detail}")
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads and types the library once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `make -C {_HERE}` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(path)
+    P = C.POINTER
+    vp = C.c_void_p
+    fp = P(C.c_float)
+    L.tsdf_default_params.argtypes = [P(TsdfParams)]
+    L.tsdf_default_params.restype = None
+    L.tsdf_create.argtypes = [P(TsdfParams), P(vp)]
+    L.tsdf_destroy.argtypes = [vp]
+    L.tsdf_destroy.restype = None
+    L.tsdf_reset.argtypes = [vp, vp]
+    L.tsdf_set_intrinsics.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.tsdf_integrate.argtypes = [vp, vp, C.c_int32, vp, fp, C.c_int32, vp]
+    L.tsdf_integrate_batch.argtypes = [vp, C.c_int32, P(vp), C.c_int32, P(vp), fp, P(C.c_int32), vp]
+    L.tsdf_reintegrate_batch.argtypes = [vp, C.c_int32, P(vp), C.c_int32, P(vp), fp, fp, vp]
+    L.tsdf_raycast.argtypes = [vp, fp, vp, vp, vp, vp, vp]
+    L.tsdf_sync.argtypes = [vp, vp]
+    L.tsdf_stats.argtypes = [vp, P(TsdfStats), vp]
+    L.tsdf_export_bricks.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, P(C.c_uint64), vp]
+    L.tsdf_pose_prepare.argtypes = [fp, fp, fp]
+    L.tsdf_status_string.argtypes = [C.c_int]
+    L.tsdf_status_string.restype = C.c_char_p
+    L.tsdf_last_error.argtypes = []
+    L.tsdf_last_error.restype = C.c_char_p
+    for name in ("tsdf_create", "tsdf_reset", "tsdf_set_intrinsics", "tsdf_integrate",
+                 "tsdf_integrate_batch", "tsdf_reintegrate_batch", "tsdf_raycast",
+                 "tsdf_sync", "tsdf_stats", "tsdf_export_bricks", "tsdf_pose_prepare"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(status: int, where: str) -> None:
+    if status != TSDF_OK:
+        L = load()
+        raise TsdfError(status, where, L.tsdf_last_error().decode(errors="replace"))

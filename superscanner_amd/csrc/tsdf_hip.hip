@@ -1,0 +1,1441 @@
+// tsdf_hip.hip -- MI355X (gfx950) voxel-hashed TSDF engine behind the C ABI of
+// include/tsdf.h.
+//
+// Device data layout (all in HBM, owned by the handle):
+//   slots      [n_buckets] u64   open-addressing hash, (key << 32) | brick id
+//   slot_epoch [n_buckets] u32   call stamp; replaces the reference's
+//                                alreadyIntegrated flag + O(capacity) reset
+//                                (GPUVoxelGrid.cpp:76-78, 552-559)
+//   coords     [n_bricks] int4   block index (x, y, z) + flags (bit 0: fresh)
+//   payload    [n_bricks][2][512] T   per brick: sdf[512] then w[512]
+//                                (T = float or half; voxel v = x + 8y + 64z)
+//   rgb        [n_bricks][3][512] float   optional colour, channel-planar
+//   touched    [n_buckets] u32   slots claimed by the current call
+//   ptmap      [H][W] float4     world point of every pixel of the current
+//                                frame, w = 1 where camera z > rmin
+//
+// Kernels per integrate call: k_collect (allocation, one pass over pixels)
+// and k_integrate (one wavefront per touched brick).  The brick count never
+// leaves the device: k_integrate is a persistent grid that reads it.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/tsdf.h"
+#include "tsdf_math.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace tsdf;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// device-side state
+// ---------------------------------------------------------------------------
+
+struct Ctrl {
+    unsigned int alloc_top;         // brick ids handed out (may pass n_bricks)
+    unsigned int overflow;          // refused allocations / full table
+    unsigned int touched_count[2];  // touched-list length, by call parity
+    unsigned long long total_touched;
+    unsigned long long total_new;
+    unsigned long long total_updated;
+    unsigned long long total_valid;
+    unsigned long long ray_hits;
+    unsigned long long ray_ub;
+    unsigned long long ray_samples;
+    unsigned long long ray_probes;
+};
+
+struct GridView {
+    unsigned long long* slots;
+    uint32_t* slot_epoch;
+    int4* coords;
+    void* payload;
+    float* rgb;
+    uint32_t* touched;
+    Ctrl* ctrl;
+    uint32_t mask;      // n_buckets - 1
+    uint32_t n_bricks;
+    float vl, vbl, trunc, rmin, dmax, wmax;
+    int H, W;
+    Camera cam;
+};
+
+// One integrate (+1) or deintegrate (-1) operation.
+struct OpDesc {
+    const uint16_t* depth;
+    const uint8_t* rgb;
+    float4* ptmap;
+    int pitch;
+    int sign;
+    Pose pose;
+};
+
+// (v + 0.5) * vl in double for v = 0..7 (the reference's promotion at
+// GPUVoxelGrid.cpp:521).
+struct Offsets {
+    double v[8];
+};
+
+constexpr int kThreads = 256;
+constexpr int kTileW = 64;      // collection tile: 64 x 16 pixels, 4 per thread
+constexpr int kTileH = 16;
+constexpr int kLdsSet = 2048;   // per-tile block set in LDS (16 KiB)
+constexpr int kMaxOps = 256;    // ops per batched pass
+constexpr int kMaskWords = kMaxOps / 32;
+constexpr int kMaxAxisBlocks = 64;
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float ld(const float* p) { return *p; }
+__device__ __forceinline__ float ld(const __half* p) { return __half2float(*p); }
+__device__ __forceinline__ void st(float* p, float v) { *p = v; }
+__device__ __forceinline__ void st(__half* p, float v) { *p = __float2half_rn(v); }
+
+__device__ __forceinline__ unsigned long long pack_coord(int x, int y, int z)
+{
+    return ((unsigned long long)(x & 0x1FFFFF) << 42) | ((unsigned long long)(y & 0x1FFFFF) << 21) |
+           (unsigned long long)(z & 0x1FFFFF);
+}
+__device__ __forceinline__ int unpack21(unsigned long long v)
+{
+    int r = (int)(v & 0x1FFFFF);
+    return (r ^ 0x100000) - 0x100000;
+}
+__device__ __forceinline__ bool packable(int v) { return v >= -(1 << 20) && v < (1 << 20); }
+
+// Appends `val` for every lane with `need` using one atomic per wavefront.
+__device__ __forceinline__ void wave_append(bool need, uint32_t val, unsigned int* counter,
+                                            uint32_t* list)
+{
+    const unsigned long long m = __ballot(need);
+    if (m == 0)
+        return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0;
+    if (lane == leader)
+        base = atomicAdd(counter, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    if (need)
+        list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
+}
+
+__device__ __forceinline__ unsigned int wave_sum(unsigned int v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
+// Finds `key` or inserts it.  The thread whose CAS claims an empty slot
+// takes the next brick id and publishes it; concurrent finders of the same
+// key only need the slot index.  Returns the slot or -1 if the table is full.
+__device__ int find_or_insert(const GridView& g, uint32_t key, int x, int y, int z)
+{
+    uint32_t b = mix32(key) & g.mask;
+    const unsigned long long keyhi = (unsigned long long)key << 32;
+    for (uint32_t p = 0; p <= g.mask; ++p) {
+        unsigned long long s = __hip_atomic_load(&g.slots[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s == kSlotEmpty) {
+            const unsigned long long old = atomicCAS(&g.slots[b], kSlotEmpty, keyhi | kIdPending);
+            if (old == kSlotEmpty) {
+                const unsigned int id = atomicAdd(&g.ctrl->alloc_top, 1u);
+                unsigned int idv = kIdOverflow;
+                if (id < g.n_bricks) {
+                    g.coords[id] = make_int4(x, y, z, 1);
+                    idv = id;
+                    atomicAdd(&g.ctrl->total_new, 1ull);
+                } else {
+                    atomicAdd(&g.ctrl->overflow, 1u);
+                }
+                atomicExch(&g.slots[b], keyhi | idv);
+                return (int)b;
+            }
+            s = old;
+        }
+        if ((uint32_t)(s >> 32) == key)
+            return (int)b;
+        b = (b + 1) & g.mask;
+    }
+    return -1;
+}
+
+// Brick id of block (x, y, z) or -1 (voxelMap2.find + valid_at).
+__device__ __forceinline__ int find_brick(const GridView& g, int x, int y, int z)
+{
+    const uint32_t key = prime_hash(x, y, z);
+    uint32_t b = mix32(key) & g.mask;
+    for (uint32_t p = 0; p <= g.mask; ++p) {
+        const unsigned long long s = g.slots[b];
+        if (s == kSlotEmpty)
+            return -1;
+        if ((uint32_t)(s >> 32) == key) {
+            const uint32_t id = (uint32_t)s;
+            return id < g.n_bricks ? (int)id : -1;
+        }
+        b = (b + 1) & g.mask;
+    }
+    return -1;
+}
+
+// Marks the block as touched by this call (and by op `op` in a batch).
+// Returns true for the one caller that must append the slot to the list.
+template <bool kBatch>
+__device__ __forceinline__ bool claim(const GridView& g, int x, int y, int z, uint32_t epoch,
+                                      uint32_t* op_mask, int op, uint32_t* slot_out)
+{
+    const int s = find_or_insert(g, prime_hash(x, y, z), x, y, z);
+    if (s < 0) {
+        atomicAdd(&g.ctrl->overflow, 1u);
+        return false;
+    }
+    if (kBatch)
+        atomicOr(&op_mask[(size_t)s * kMaskWords + (op >> 5)], 1u << (op & 31));
+    *slot_out = (uint32_t)s;
+    return atomicExch(&g.slot_epoch[s], epoch) != epoch;
+}
+
+__device__ __forceinline__ bool lds_insert(unsigned long long* set, unsigned long long v)
+{
+    uint32_t h = mix32((uint32_t)v ^ ((uint32_t)(v >> 32) * 0x9E3779B1u)) & (kLdsSet - 1);
+    for (int p = 0; p < 32; ++p) {
+        const unsigned long long old = atomicCAS(&set[h], kSlotEmpty, v);
+        if (old == kSlotEmpty || old == v)
+            return true;
+        h = (h + 1) & (kLdsSet - 1);
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// k_reset: empty table, zero counters
+// ---------------------------------------------------------------------------
+
+__global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
+{
+    const size_t n = (size_t)g.mask + 1;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        g.slots[i] = kSlotEmpty;
+        g.slot_epoch[i] = 0u;
+    }
+    if (op_mask)
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < op_mask_words; i += stride)
+            op_mask[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Ctrl z;
+        memset(&z, 0, sizeof(z));
+        *g.ctrl = z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_collect: block allocation (vbCollectionTag, GPUVoxelGrid.cpp:471-514)
+// ---------------------------------------------------------------------------
+// One workgroup per 64 x 16 pixel tile.  Each valid pixel enumerates the
+// blocks of its +-trunc box into an LDS set, so the global table sees each
+// block once per tile; then the tile's set is claimed against the global
+// table with wavefront-aggregated appends to the touched list.  The pixel's
+// world point goes to ptmap for k_integrate (the reference's inputPcd,
+// GPUVoxelGrid.h:303-307, transformed as at GPUVoxelGrid.cpp:356).
+template <bool kBatch>
+__device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch, int par,
+                             uint32_t* op_mask, int op_index)
+{
+    __shared__ unsigned long long lset[kLdsSet];
+    __shared__ unsigned int s_valid;
+    for (int e = threadIdx.x; e < kLdsSet; e += kThreads)
+        lset[e] = kSlotEmpty;
+    if (threadIdx.x == 0)
+        s_valid = 0;
+    __syncthreads();
+
+    const int tx = threadIdx.x & (kTileW - 1);
+    const int ty = threadIdx.x / kTileW;
+    const int j = blockIdx.x * kTileW + tx;
+    unsigned int nvalid = 0;
+    for (int r = 0; r < kTileH / (kThreads / kTileW); ++r) {
+        const int i = blockIdx.y * kTileH + ty + r * (kThreads / kTileW);
+        if (i >= g.H || j >= g.W)
+            continue;
+        const float d = (float)op.depth[(size_t)i * op.pitch + j] / 1000.0f;
+        const f3 p = get_point(g.cam, i, j, d);
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p.z > g.rmin) {
+            ++nvalid;
+            const f3 pt = xform(p, op.pose.E);
+            o = make_float4(pt.x, pt.y, pt.z, 1.0f);
+            const int x0 = vb_index(pt.x - g.trunc, g.vbl), x1 = vb_index(pt.x + g.trunc, g.vbl);
+            const int y0 = vb_index(pt.y - g.trunc, g.vbl), y1 = vb_index(pt.y + g.trunc, g.vbl);
+            const int z0 = vb_index(pt.z - g.trunc, g.vbl), z1 = vb_index(pt.z + g.trunc, g.vbl);
+            const bool sane = (long long)x1 - x0 < kMaxAxisBlocks && (long long)y1 - y0 < kMaxAxisBlocks &&
+                              (long long)z1 - z0 < kMaxAxisBlocks;
+            if (sane) {
+                for (int x = x0; x <= x1; ++x)
+                    for (int y = y0; y <= y1; ++y)
+                        for (int z = z0; z <= z1; ++z) {
+                            const bool ok = packable(x) && packable(y) && packable(z) &&
+                                            lds_insert(lset, pack_coord(x, y, z));
+                            if (!ok) {  // set full or far block: claim directly
+                                uint32_t slot;
+                                if (claim<kBatch>(g, x, y, z, epoch, op_mask, op_index, &slot))
+                                    g.touched[atomicAdd(&g.ctrl->touched_count[par], 1u)] = slot;
+                            }
+                        }
+            } else {
+                atomicAdd(&g.ctrl->overflow, 1u);
+            }
+        }
+        op.ptmap[(size_t)i * g.W + j] = o;
+    }
+    nvalid = wave_sum(nvalid);
+    if (__lane_id() == 0 && nvalid)
+        atomicAdd(&s_valid, nvalid);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_valid)
+        atomicAdd(&g.ctrl->total_valid, (unsigned long long)s_valid);
+
+    for (int e = threadIdx.x; e < kLdsSet; e += kThreads) {
+        const unsigned long long v = lset[e];
+        bool need = false;
+        uint32_t slot = 0;
+        if (v != kSlotEmpty)
+            need = claim<kBatch>(g, unpack21(v >> 42), unpack21(v >> 21), unpack21(v), epoch, op_mask,
+                                 op_index, &slot);
+        wave_append(need, slot, &g.ctrl->touched_count[par], g.touched);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_collect(GridView g, OpDesc op, uint32_t epoch, int par)
+{
+    collect_tile<false>(g, op, epoch, par, nullptr, 0);
+}
+
+__global__ __launch_bounds__(kThreads) void k_collect_batch(GridView g, const OpDesc* ops, uint32_t epoch,
+                                                            int par, uint32_t* op_mask)
+{
+    collect_tile<true>(g, ops[blockIdx.z], epoch, par, op_mask, blockIdx.z);
+}
+
+// ---------------------------------------------------------------------------
+// per-voxel update (updateSingleVoxel[Color] / deUpdateSingleVoxel[Color],
+// GPUVoxelGrid.cpp:345-469) on register-resident values
+// ---------------------------------------------------------------------------
+
+template <bool kColor>
+__device__ __forceinline__ bool update_voxel(const GridView& g, const OpDesc& op, float vmx, float vmy,
+                                             float vmz, float a0, float a1, float a2, float& s, float& w,
+                                             float* c)
+{
+    const float* Ei = op.pose.Ei;
+    // transformPoint(vm, E_inv) with the z-independent part hoisted:
+    // ((x*E0 + y*E1) + z*E2) + E3 keeps the reference's association.
+    const float cx = a0 + vmz * Ei[2] + Ei[3];
+    const float cy = a1 + vmz * Ei[6] + Ei[7];
+    const float cz = a2 + vmz * Ei[10] + Ei[11];
+    const float row = cy * g.cam.fy / cz + g.cam.ci;  // getCoordinate, GPUVoxelGrid.h:234-237
+    const float col = cx * g.cam.fx / cz + g.cam.cj;
+    const int cori = f2i(row), corj = f2i(col);
+    if (!(cori < g.H && corj < g.W && cori >= 0 && corj >= 0))
+        return false;
+    const float4 pt = op.ptmap[(size_t)cori * g.W + corj];
+    if (pt.w == 0.0f)  // inputPcd(cori, corj).z <= rmin
+        return false;
+    const float sdf = (pt.x - vmx) * op.pose.Ez[0] + (pt.y - vmy) * op.pose.Ez[1] + (pt.z - vmz) * op.pose.Ez[2];
+    if (!(fabsf(sdf) < g.trunc))
+        return false;
+    float px[3] = {0.f, 0.f, 0.f};
+    if (kColor && op.rgb) {
+        const uint8_t* q = op.rgb + ((size_t)cori * g.W + corj) * 3;
+        px[0] = (float)q[0];
+        px[1] = (float)q[1];
+        px[2] = (float)q[2];
+    }
+    if (op.sign > 0) {
+        s = (s * w + sdf) / (w + 1.0f);
+        if (kColor && op.rgb) {
+            c[0] = (c[0] * w + px[0]) / (w + 1.0f);
+            c[1] = (c[1] * w + px[1]) / (w + 1.0f);
+            c[2] = (c[2] * w + px[2]) / (w + 1.0f);
+        }
+        // Weight cap: project semantics, inactive at max_weight == 0 (the
+        // reference never caps, GPUVoxelGrid.h:14,62).
+        w = (g.wmax > 0.0f && w >= g.wmax) ? g.wmax : w + 1.0f;
+    } else {
+        if (w != 1.0f) {
+            s = (s * w - sdf) / (w - 1.0f);
+            if (kColor && op.rgb) {
+                c[0] = (c[0] * w - px[0]) / (w - 1.0f);
+                c[1] = (c[1] * w - px[1]) / (w - 1.0f);
+                c[2] = (c[2] * w - px[2]) / (w - 1.0f);
+            }
+            w = w - 1.0f;
+        } else {
+            w = 0.0f;
+            s = 0.0f;
+            if (kColor && op.rgb)
+                c[0] = c[1] = c[2] = 0.0f;
+        }
+    }
+    return true;
+}
+
+// Which touched-list entries a wavefront handles.  Workgroups are dealt to
+// the 8 XCDs round-robin; each XCD gets one contiguous eighth of the list so
+// the bricks (and ptmap pixels) that one XCD's L2 serves are spatial
+// neighbours.
+struct WaveRange {
+    unsigned int begin, end, stride;
+};
+
+__device__ __forceinline__ WaveRange wave_range(unsigned int count)
+{
+    const unsigned int xcd = blockIdx.x & 7u;
+    const unsigned int per_xcd_wg = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    const unsigned int local_wave = (blockIdx.x >> 3) * (kThreads / 64) + (threadIdx.x >> 6);
+    const unsigned int seg = (count + 7u) >> 3;
+    const unsigned int sb = min(count, xcd * seg), se = min(count, sb + seg);
+    WaveRange r;
+    r.begin = sb + local_wave;
+    r.end = se;
+    r.stride = per_xcd_wg * (kThreads / 64);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// k_integrate: integrationCore[Color]Tag / deIntegrationCore[Color]Tag,
+// GPUVoxelGrid.cpp:516-550.  One wavefront per touched brick, lane = (x, y),
+// eight voxels along z per lane.
+// ---------------------------------------------------------------------------
+template <typename T, bool kColor>
+__global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, int par, Offsets off)
+{
+    const unsigned int count = g.ctrl->touched_count[par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g.ctrl->touched_count[par ^ 1] = 0u;  // ready for the next call
+        g.ctrl->total_touched += count;
+    }
+    const int lane = threadIdx.x & 63;
+    const int x = lane & 7, y = lane >> 3;
+    unsigned int nupd = 0;
+    const WaveRange wr = wave_range(count);
+    for (unsigned int t = wr.begin; t < wr.end; t += wr.stride) {
+        const uint32_t id = (uint32_t)g.slots[g.touched[t]];
+        if (id >= g.n_bricks)
+            continue;
+        const int4 c = g.coords[id];
+        const bool fresh = (c.w & 1) != 0;
+        if (fresh && lane == 0)
+            g.coords[id].w = c.w & ~1;
+        const float pzx = g.vbl * (float)c.x, pzy = g.vbl * (float)c.y, pzz = g.vbl * (float)c.z;
+        const float vmx = (float)((double)pzx + off.v[x]);
+        const float vmy = (float)((double)pzy + off.v[y]);
+        const float* Ei = op.pose.Ei;
+        const float a0 = vmx * Ei[0] + vmy * Ei[1];
+        const float a1 = vmx * Ei[4] + vmy * Ei[5];
+        const float a2 = vmx * Ei[8] + vmy * Ei[9];
+        T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
+        T* wd = sd + kVPB;
+        float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
+#pragma unroll 2
+        for (int z = 0; z < kVPE; ++z) {
+            const int v = lane + 64 * z;
+            const float vmz = (float)((double)pzz + off.v[z]);
+            float s = 0.f, w = 0.f, col[3] = {0.f, 0.f, 0.f};
+            if (!fresh) {
+                s = ld(sd + v);
+                w = ld(wd + v);
+                if (kColor) {
+                    col[0] = cr[v];
+                    col[1] = cr[v + kVPB];
+                    col[2] = cr[v + 2 * kVPB];
+                }
+            }
+            const bool upd = update_voxel<kColor>(g, op, vmx, vmy, vmz, a0, a1, a2, s, w, col);
+            if (upd || fresh) {
+                st(sd + v, s);
+                st(wd + v, w);
+                if (kColor) {
+                    cr[v] = col[0];
+                    cr[v + kVPB] = col[1];
+                    cr[v + 2 * kVPB] = col[2];
+                }
+            }
+            nupd += upd ? 1u : 0u;
+        }
+    }
+    nupd = wave_sum(nupd);
+    if (lane == 0 && nupd)
+        atomicAdd(&g.ctrl->total_updated, (unsigned long long)nupd);
+}
+
+// ---------------------------------------------------------------------------
+// k_integrate_batch: every op of a batch applied to one brick while its
+// voxels stay in registers; ops run in their original order, so the result
+// equals the sequence of plain calls bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T, bool kColor>
+__global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const OpDesc* ops, int par,
+                                                              Offsets off, uint32_t* op_mask)
+{
+    const unsigned int count = g.ctrl->touched_count[par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g.ctrl->touched_count[par ^ 1] = 0u;
+        g.ctrl->total_touched += count;
+    }
+    const int lane = threadIdx.x & 63;
+    const int x = lane & 7, y = lane >> 3;
+    unsigned int nupd = 0;
+    const WaveRange wr = wave_range(count);
+    for (unsigned int t = wr.begin; t < wr.end; t += wr.stride) {
+        const uint32_t slot = g.touched[t];
+        uint32_t words[kMaskWords];
+#pragma unroll
+        for (int k = 0; k < kMaskWords; ++k)
+            words[k] = op_mask[(size_t)slot * kMaskWords + k];
+        if (lane < kMaskWords)
+            op_mask[(size_t)slot * kMaskWords + lane] = 0u;
+        const uint32_t id = (uint32_t)g.slots[slot];
+        if (id >= g.n_bricks)
+            continue;
+        const int4 c = g.coords[id];
+        const bool fresh = (c.w & 1) != 0;
+        if (fresh && lane == 0)
+            g.coords[id].w = c.w & ~1;
+        const float pzx = g.vbl * (float)c.x, pzy = g.vbl * (float)c.y, pzz = g.vbl * (float)c.z;
+        const float vmx = (float)((double)pzx + off.v[x]);
+        const float vmy = (float)((double)pzy + off.v[y]);
+        T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
+        T* wd = sd + kVPB;
+        float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
+        float s[kVPE], w[kVPE], col[kVPE][3];
+        float vmz[kVPE];
+        bool dirty[kVPE];
+#pragma unroll
+        for (int z = 0; z < kVPE; ++z) {
+            const int v = lane + 64 * z;
+            vmz[z] = (float)((double)pzz + off.v[z]);
+            dirty[z] = fresh;
+            s[z] = fresh ? 0.f : ld(sd + v);
+            w[z] = fresh ? 0.f : ld(wd + v);
+            col[z][0] = col[z][1] = col[z][2] = 0.f;
+            if (kColor && !fresh) {
+                col[z][0] = cr[v];
+                col[z][1] = cr[v + kVPB];
+                col[z][2] = cr[v + 2 * kVPB];
+            }
+        }
+        for (int k = 0; k < kMaskWords; ++k) {
+            uint32_t bits = words[k];
+            while (bits) {
+                const int b = __ffs(bits) - 1;
+                bits &= bits - 1u;
+                const OpDesc& op = ops[k * 32 + b];
+                const float* Ei = op.pose.Ei;
+                const float a0 = vmx * Ei[0] + vmy * Ei[1];
+                const float a1 = vmx * Ei[4] + vmy * Ei[5];
+                const float a2 = vmx * Ei[8] + vmy * Ei[9];
+#pragma unroll
+                for (int z = 0; z < kVPE; ++z) {
+                    const bool u = update_voxel<kColor>(g, op, vmx, vmy, vmz[z], a0, a1, a2, s[z], w[z], col[z]);
+                    dirty[z] = dirty[z] || u;
+                    nupd += u ? 1u : 0u;
+                }
+            }
+        }
+#pragma unroll
+        for (int z = 0; z < kVPE; ++z) {
+            if (!dirty[z])
+                continue;
+            const int v = lane + 64 * z;
+            st(sd + v, s[z]);
+            st(wd + v, w[z]);
+            if (kColor) {
+                cr[v] = col[z][0];
+                cr[v + kVPB] = col[z][1];
+                cr[v + 2 * kVPB] = col[z][2];
+            }
+        }
+    }
+    nupd = wave_sum(nupd);
+    if (lane == 0 && nupd)
+        atomicAdd(&g.ctrl->total_updated, (unsigned long long)nupd);
+}
+
+// ---------------------------------------------------------------------------
+// k_raycast: raycastingTag, GPUVoxelGrid.cpp:561-715.  One lane per pixel,
+// 8 x 8 pixels per wavefront, 16 x 16 per workgroup.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Vox {
+    float s, w;
+};
+
+template <typename T>
+__device__ __forceinline__ Vox<T> read_voxel(const GridView& g, int id, int vx, int vy, int vz)
+{
+    const T* sd = (const T*)g.payload + (size_t)id * (2 * kVPB);
+    const int v = vx + kVPE * vy + kVPE * kVPE * vz;
+    Vox<T> r;
+    r.s = ld(sd + v);
+    r.w = ld(sd + kVPB + v);
+    return r;
+}
+
+__device__ __forceinline__ bool vin_ok(int x, int y, int z)
+{
+    return (unsigned)x < (unsigned)kVPE && (unsigned)y < (unsigned)kVPE && (unsigned)z < (unsigned)kVPE;
+}
+
+struct RayOut {
+    f3 p, c;
+    int code;  // 1 hit, 0 miss, -flag UB
+    unsigned int samples, probes;
+};
+
+template <typename T, bool kColor>
+__device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
+{
+    RayOut out;
+    out.p = {0.f, 0.f, 0.f};
+    out.c = {0.f, 0.f, 0.f};
+    out.code = 0;
+    out.samples = 0;
+    out.probes = 0;
+    const float vl = g.vl, vbl = g.vbl, dt = g.trunc, dmax = g.dmax;
+    // createNormPcdTag, GPUVoxelGrid.cpp:717-722
+    f3 n = get_point(g.cam, i, j, 1.0f);
+    const float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+    n.x /= len;
+    n.y /= len;
+    n.z /= len;
+    f3 rv = xform(n, pose.E);
+    const f3 dir = {rv.x - pose.E[3], rv.y - pose.E[7], rv.z - pose.E[11]};
+    float l = 0.1f;
+    const float a = 1.0f - l;
+    rv.x -= a * dir.x;
+    rv.y -= a * dir.y;
+    rv.z -= a * dir.z;
+
+    bool done = false, Fn = false, Lp = false;
+    int ox = 0, oy = 0, oz = 0;
+    while (!done && !Fn && l < dmax) {
+        int bx = vb_index(rv.x, vbl), by = vb_index(rv.y, vbl), bz = vb_index(rv.z, vbl);
+        int id = find_brick(g, bx, by, bz);
+        ++out.probes;
+        int count = 0;
+        while (id < 0 && l < dmax) {
+            rv.x += vbl * dir.x;
+            rv.y += vbl * dir.y;
+            rv.z += vbl * dir.z;
+            l += vbl;
+            bx = vb_index(rv.x, vbl);
+            by = vb_index(rv.y, vbl);
+            bz = vb_index(rv.z, vbl);
+            id = find_brick(g, bx, by, bz);
+            ++out.probes;
+            ++count;
+        }
+        if (l >= dmax)
+            break;
+        if (count > 0) {
+            const float back = vbl - dt;
+            rv.x -= back * dir.x;
+            rv.y -= back * dir.y;
+            rv.z -= back * dir.z;
+            l -= back;
+            bx = vb_index(rv.x, vbl);
+            by = vb_index(rv.y, vbl);
+            bz = vb_index(rv.z, vbl);
+            id = find_brick(g, bx, by, bz);
+            ++out.probes;
+            while (id < 0 && l < dmax) {
+                rv.x += dt * dir.x;
+                rv.y += dt * dir.y;
+                rv.z += dt * dir.z;
+                l += dt;
+                bx = vb_index(rv.x, vbl);
+                by = vb_index(rv.y, vbl);
+                bz = vb_index(rv.z, vbl);
+                id = find_brick(g, bx, by, bz);
+                ++out.probes;
+            }
+        }
+        if (l >= dmax)
+            break;
+        const int cur = id;
+        int vx = vin_index(rv.x, bx, vbl, vl), vy = vin_index(rv.y, by, vbl, vl), vz = vin_index(rv.z, bz, vbl, vl);
+        ox = bx;
+        oy = by;
+        oz = bz;
+        while (!done && l < dmax && ox == bx && oy == by && oz == bz && !Fn) {
+            if (!vin_ok(vx, vy, vz)) {
+                out.code = -TSDF_RAY_UB_VOXEL_INDEX;
+                return out;
+            }
+            const Vox<T> cv = read_voxel<T>(g, cur, vx, vy, vz);
+            ++out.samples;
+            if (cv.w > 0) {
+                if (cv.s > 0) {
+                    Lp = true;
+                    rv.x += vl * dir.x;
+                    rv.y += vl * dir.y;
+                    rv.z += vl * dir.z;
+                    l += vl;
+                    ox = bx;
+                    oy = by;
+                    oz = bz;
+                    bx = vb_index(rv.x, vbl);
+                    by = vb_index(rv.y, vbl);
+                    bz = vb_index(rv.z, vbl);
+                    vx = vin_index(rv.x, bx, vbl, vl);
+                    vy = vin_index(rv.y, by, vbl, vl);
+                    vz = vin_index(rv.z, bz, vbl, vl);
+                } else if (Lp) {
+                    done = true;
+                    const f3 vm_in = {voxel_mid(vx, bx, vbl, vl), voxel_mid(vy, by, vbl, vl),
+                                      voxel_mid(vz, bz, vbl, vl)};
+                    const int ivx = vx, ivy = vy, ivz = vz;
+                    rv.x -= vl * dir.x;
+                    rv.y -= vl * dir.y;
+                    rv.z -= vl * dir.z;
+                    l -= vl;
+                    bx = vb_index(rv.x, vbl);
+                    by = vb_index(rv.y, vbl);
+                    bz = vb_index(rv.z, vbl);
+                    vx = vin_index(rv.x, bx, vbl, vl);
+                    vy = vin_index(rv.y, by, vbl, vl);
+                    vz = vin_index(rv.z, bz, vbl, vl);
+                    const f3 vm_out = {voxel_mid(vx, bx, vbl, vl), voxel_mid(vy, by, vbl, vl),
+                                       voxel_mid(vz, bz, vbl, vl)};
+                    int ob = cur;
+                    if (bx != ox || by != oy || bz != oz) {
+                        ob = find_brick(g, bx, by, bz);
+                        ++out.probes;
+                        if (ob < 0) {
+                            out.code = -TSDF_RAY_UB_MISSING_OUT;
+                            return out;
+                        }
+                    }
+                    if (!vin_ok(vx, vy, vz)) {
+                        out.code = -TSDF_RAY_UB_VOXEL_INDEX;
+                        return out;
+                    }
+                    const Vox<T> vo = read_voxel<T>(g, ob, vx, vy, vz);
+                    ++out.samples;
+                    const float xx = (vo.w * vo.s / vl + cv.w * (1.0f - cv.s / vl)) / (cv.w + vo.w);
+                    f3 p;
+                    p.x = vm_out.x + xx * (vm_in.x - vm_out.x);
+                    p.y = vm_out.y + xx * (vm_in.y - vm_out.y);
+                    p.z = vm_out.z + xx * (vm_in.z - vm_out.z);
+                    const float sc = dot3(p, dir) - dot3(rv, dir);
+                    out.p.x = rv.x + sc * dir.x;
+                    out.p.y = rv.y + sc * dir.y;
+                    out.p.z = rv.z + sc * dir.z;
+                    if (kColor) {
+                        const float* co = g.rgb + (size_t)ob * (3 * kVPB);
+                        const float* cc = g.rgb + (size_t)cur * (3 * kVPB);
+                        const int vo_i = vx + kVPE * vy + kVPE * kVPE * vz;
+                        const int vi_i = ivx + kVPE * ivy + kVPE * kVPE * ivz;
+                        out.c.x = (float)((0.5 * (double)co[vo_i] + 0.5 * (double)cc[vi_i]) / 255.0);
+                        out.c.y = (float)((0.5 * (double)co[vo_i + kVPB] + 0.5 * (double)cc[vi_i + kVPB]) / 255.0);
+                        out.c.z =
+                            (float)((0.5 * (double)co[vo_i + 2 * kVPB] + 0.5 * (double)cc[vi_i + 2 * kVPB]) / 255.0);
+                    }
+                } else {
+                    Fn = true;
+                }
+            } else {
+                Lp = false;
+                rv.x += vl * dir.x;
+                rv.y += vl * dir.y;
+                rv.z += vl * dir.z;
+                l += vl;
+                ox = bx;
+                oy = by;
+                oz = bz;
+                bx = vb_index(rv.x, vbl);
+                by = vb_index(rv.y, vbl);
+                bz = vb_index(rv.z, vbl);
+                vx = vin_index(rv.x, bx, vbl, vl);
+                vy = vin_index(rv.y, by, vbl, vl);
+                vz = vin_index(rv.z, bz, vbl, vl);
+            }
+        }
+    }
+    out.code = done ? 1 : 0;
+    return out;
+}
+
+template <typename T, bool kColor>
+__global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, float* points, float* colors,
+                                                      float* depth, uint8_t* ub, int tiles_x, int n_tiles)
+{
+    __shared__ unsigned int s_cnt[4];
+    if (threadIdx.x < 4)
+        s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    // XCD-aware tile order: workgroups b, b+8, b+16, ... share an XCD and get
+    // consecutive tiles, so one XCD's L2 serves one band of the image.
+    const int per = (n_tiles + 7) >> 3;
+    const int tile = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = (tile / tiles_x) * 16 + (wv >> 1) * 8 + (lane >> 3);
+    const int j = (tile % tiles_x) * 16 + (wv & 1) * 8 + (lane & 7);
+    if (tile < n_tiles && i < g.H && j < g.W) {
+        const RayOut r = march<T, kColor>(g, pose, i, j);
+        const size_t o = (size_t)i * g.W + j;
+        f3 p = r.p, c = r.c;
+        if (r.code < 0) {
+            p = {0.f, 0.f, 0.f};
+            c = {0.f, 0.f, 0.f};
+        }
+        points[o * 3 + 0] = p.x;
+        points[o * 3 + 1] = p.y;
+        points[o * 3 + 2] = p.z;
+        if (colors) {
+            colors[o * 3 + 0] = c.x;
+            colors[o * 3 + 1] = c.y;
+            colors[o * 3 + 2] = c.z;
+        }
+        if (depth)
+            depth[o] = (r.code > 0) ? (p.x * pose.Ei[8] + p.y * pose.Ei[9] + p.z * pose.Ei[10] + pose.Ei[11]) : 0.0f;
+        if (ub)
+            ub[o] = (uint8_t)(r.code < 0 ? -r.code : 0);
+        if (r.code > 0)
+            atomicAdd(&s_cnt[0], 1u);
+        if (r.code < 0)
+            atomicAdd(&s_cnt[1], 1u);
+        atomicAdd(&s_cnt[2], r.samples);
+        atomicAdd(&s_cnt[3], r.probes);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_cnt[0]) atomicAdd(&g.ctrl->ray_hits, (unsigned long long)s_cnt[0]);
+        if (s_cnt[1]) atomicAdd(&g.ctrl->ray_ub, (unsigned long long)s_cnt[1]);
+        if (s_cnt[2]) atomicAdd(&g.ctrl->ray_samples, (unsigned long long)s_cnt[2]);
+        if (s_cnt[3]) atomicAdd(&g.ctrl->ray_probes, (unsigned long long)s_cnt[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_export: bricks -> caller buffers in fp32 (one workgroup per brick)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_export(GridView g, unsigned int n, int32_t* coords, float* sdf,
+                                                     float* w, float* rgb)
+{
+    for (unsigned int b = blockIdx.x; b < n; b += gridDim.x) {
+        const T* sd = (const T*)g.payload + (size_t)b * (2 * kVPB);
+        if (threadIdx.x == 0 && coords) {
+            const int4 c = g.coords[b];
+            coords[(size_t)b * 3 + 0] = c.x;
+            coords[(size_t)b * 3 + 1] = c.y;
+            coords[(size_t)b * 3 + 2] = c.z;
+        }
+        const bool fresh = (g.coords[b].w & 1) != 0;  // allocated, never written
+        for (int v = threadIdx.x; v < kVPB; v += kThreads) {
+            if (sdf) sdf[(size_t)b * kVPB + v] = fresh ? 0.f : ld(sd + v);
+            if (w) w[(size_t)b * kVPB + v] = fresh ? 0.f : ld(sd + kVPB + v);
+            if (rgb && g.rgb)
+                for (int ch = 0; ch < 3; ++ch)
+                    rgb[((size_t)b * kVPB + v) * 3 + ch] = fresh ? 0.f : g.rgb[(size_t)b * 3 * kVPB + ch * kVPB + v];
+        }
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+
+struct tsdf_grid {
+    tsdf_params p;
+    float vbl;
+    bool intr_set;
+    Camera cam;
+    uint32_t epoch;
+    Offsets off;
+    int n_cu;
+    // device buffers
+    unsigned long long* slots;
+    uint32_t* slot_epoch;
+    int4* coords;
+    void* payload;
+    float* rgb;
+    uint32_t* touched;
+    float4* ptmap;
+    Ctrl* ctrl;
+    Ctrl* ctrl_host;  // pinned
+    // batch buffers (allocated on first batched call)
+    uint32_t* op_mask;
+    float4* ptmap_batch;
+    OpDesc* ops_dev;
+    OpDesc* ops_host;  // pinned
+};
+
+namespace {
+
+thread_local std::string t_err;
+
+tsdf_status fail(tsdf_status s, const std::string& msg)
+{
+    t_err = msg;
+    return s;
+}
+
+#define HIPC(expr)                                                                                  \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail(TSDF_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+struct DeviceScope {
+    int prev = -1;
+    bool ok = true;
+    explicit DeviceScope(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        if (prev != dev)
+            ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceScope()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev)
+            (void)hipSetDevice(prev);
+    }
+};
+
+GridView view(const tsdf_grid* g)
+{
+    GridView v;
+    v.slots = g->slots;
+    v.slot_epoch = g->slot_epoch;
+    v.coords = g->coords;
+    v.payload = g->payload;
+    v.rgb = g->rgb;
+    v.touched = g->touched;
+    v.ctrl = g->ctrl;
+    v.mask = g->p.n_buckets - 1u;
+    v.n_bricks = g->p.n_bricks;
+    v.vl = g->p.voxel_length;
+    v.vbl = g->vbl;
+    v.trunc = g->p.truncation;
+    v.rmin = g->p.min_depth;
+    v.dmax = g->p.max_ray_length;
+    v.wmax = g->p.max_weight;
+    v.H = g->p.height;
+    v.W = g->p.width;
+    v.cam = g->cam;
+    return v;
+}
+
+bool pose_ok(const float* E)
+{
+    for (int i = 0; i < 16; ++i)
+        if (!std::isfinite(E[i]))
+            return false;
+    if (E[12] != 0.0f || E[13] != 0.0f || E[14] != 0.0f || E[15] != 1.0f)
+        return false;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double s = 0;
+            for (int r = 0; r < 3; ++r)
+                s += (double)E[r * 4 + a] * (double)E[r * 4 + b];
+            if (std::fabs(s - (a == b ? 1.0 : 0.0)) > 1e-4)
+                return false;
+        }
+    for (int r = 0; r < 3; ++r)
+        if (std::fabs(E[r * 4 + 3]) > 1e5f)
+            return false;
+    return true;
+}
+
+void make_pose(const float* E, Pose* P)
+{
+    float Ei[16];
+    tsdf_pose_prepare(E, Ei, P->Ez);
+    for (int k = 0; k < 12; ++k) {
+        P->E[k] = E[k];
+        P->Ei[k] = Ei[k];
+    }
+    P->pad = 0.f;
+}
+
+int persistent_grid(const tsdf_grid* g)
+{
+    // eight resident workgroups per CU, a multiple of the 8 XCDs
+    int n = g->n_cu * 8;
+    n = (n + 7) & ~7;
+    return n < 64 ? 64 : n;
+}
+
+template <typename T>
+void launch_integrate(const tsdf_grid* g, const OpDesc& op, int par, hipStream_t s, bool color)
+{
+    const int grid = persistent_grid(g);
+    if (color)
+        hipLaunchKernelGGL((k_integrate<T, true>), dim3(grid), dim3(kThreads), 0, s, view(g), op, par, g->off);
+    else
+        hipLaunchKernelGGL((k_integrate<T, false>), dim3(grid), dim3(kThreads), 0, s, view(g), op, par, g->off);
+}
+
+template <typename T>
+void launch_integrate_batch(const tsdf_grid* g, int par, hipStream_t s, bool color)
+{
+    const int grid = persistent_grid(g);
+    if (color)
+        hipLaunchKernelGGL((k_integrate_batch<T, true>), dim3(grid), dim3(kThreads), 0, s, view(g), g->ops_dev, par,
+                           g->off, g->op_mask);
+    else
+        hipLaunchKernelGGL((k_integrate_batch<T, false>), dim3(grid), dim3(kThreads), 0, s, view(g), g->ops_dev, par,
+                           g->off, g->op_mask);
+}
+
+tsdf_status check_image_args(const tsdf_grid* g, const uint16_t* depth, int32_t pitch)
+{
+    if (!depth)
+        return fail(TSDF_BAD_ARG, "depth is NULL");
+    if (pitch < g->p.width)
+        return fail(TSDF_BAD_DIMS, "pitch smaller than the grid width");
+    return TSDF_OK;
+}
+
+tsdf_status ensure_batch_buffers(tsdf_grid* g)
+{
+    if (g->op_mask)
+        return TSDF_OK;
+    const size_t nb = g->p.n_buckets;
+    HIPC(hipMalloc(&g->op_mask, nb * kMaskWords * sizeof(uint32_t)));
+    HIPC(hipMemset(g->op_mask, 0, nb * kMaskWords * sizeof(uint32_t)));
+    HIPC(hipMalloc(&g->ptmap_batch, (size_t)kMaxOps * g->p.height * g->p.width * sizeof(float4)));
+    HIPC(hipMalloc(&g->ops_dev, kMaxOps * sizeof(OpDesc)));
+    HIPC(hipHostMalloc(&g->ops_host, kMaxOps * sizeof(OpDesc), hipHostMallocDefault));
+    HIPC(hipDeviceSynchronize());
+    return TSDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void tsdf_default_params(tsdf_params* p)
+{
+    if (!p)
+        return;
+    memset(p, 0, sizeof(*p));
+    p->voxel_length = 0.005f;
+    p->truncation = 0.02f;
+    p->height = 480;
+    p->width = 640;
+    p->n_buckets = 1u << 20;
+    p->n_bricks = 1u << 19;
+    p->storage = TSDF_STORAGE_F32;
+    p->with_color = 0;
+    p->max_weight = 0.0f;
+    p->min_depth = 0.2f;
+    p->max_ray_length = 10.0f;
+    p->device = 0;
+}
+
+const char* tsdf_status_string(tsdf_status s)
+{
+    switch (s) {
+    case TSDF_OK: return "ok";
+    case TSDF_BAD_ARG: return "bad argument";
+    case TSDF_BAD_DIMS: return "image dimensions do not match the grid";
+    case TSDF_MAP_FULL: return "brick pool or hash table full";
+    case TSDF_HIP_ERROR: return "HIP runtime error";
+    case TSDF_NO_INTRINSICS: return "intrinsics not set";
+    case TSDF_BAD_POSE: return "pose is not a finite rigid transform";
+    }
+    return "unknown status";
+}
+
+const char* tsdf_last_error(void) { return t_err.c_str(); }
+
+tsdf_status tsdf_pose_prepare(const float E[16], float Einv[16], float Ez[3])
+{
+    if (!E || !Einv || !Ez)
+        return fail(TSDF_BAD_ARG, "NULL pose pointer");
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+            Einv[r * 4 + c] = E[c * 4 + r];
+        const double t = (double)E[0 * 4 + r] * (double)E[3] + (double)E[1 * 4 + r] * (double)E[7] +
+                         (double)E[2 * 4 + r] * (double)E[11];
+        Einv[r * 4 + 3] = (float)(-t);
+    }
+    Einv[12] = 0.f;
+    Einv[13] = 0.f;
+    Einv[14] = 0.f;
+    Einv[15] = 1.f;
+    const float x = E[2], y = E[6], z = E[10];
+    const float len = std::sqrt(x * x + y * y + z * z);
+    Ez[0] = x / len;
+    Ez[1] = y / len;
+    Ez[2] = z / len;
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
+{
+    if (!p || !out)
+        return fail(TSDF_BAD_ARG, "NULL params or output handle");
+    *out = nullptr;
+    const float vl = p->voxel_length, tr = p->truncation;
+    if (!(vl >= 1e-4f && vl <= 1.0f))
+        return fail(TSDF_BAD_ARG, "voxel_length must lie in [1e-4, 1] m");
+    if (!(tr >= 1e-4f && tr <= 4.0f * kVPE * vl))
+        return fail(TSDF_BAD_ARG, "truncation must lie in [1e-4 m, 4 brick lengths]");
+    if (p->height < 1 || p->width < 1 || p->height > 16384 || p->width > 16384)
+        return fail(TSDF_BAD_DIMS, "image size out of range");
+    if (p->n_buckets < 1024u || (p->n_buckets & (p->n_buckets - 1u)) || p->n_buckets > (1u << 30))
+        return fail(TSDF_BAD_ARG, "n_buckets must be a power of two in [2^10, 2^30]");
+    if (p->n_bricks < 1u || p->n_bricks > p->n_buckets || p->n_bricks >= 0x80000000u)
+        return fail(TSDF_BAD_ARG, "n_bricks must lie in [1, n_buckets]");
+    if (p->storage != TSDF_STORAGE_F32 && p->storage != TSDF_STORAGE_F16)
+        return fail(TSDF_BAD_ARG, "unknown storage");
+    if (!(p->max_ray_length > 0.0f && p->max_ray_length <= 1000.0f) ||
+        p->max_ray_length / (vl < tr ? vl : tr) > 1e6f)
+        return fail(TSDF_BAD_ARG, "max_ray_length out of range");
+    if (!(p->min_depth >= 0.0f) || !(p->max_weight >= 0.0f))
+        return fail(TSDF_BAD_ARG, "min_depth and max_weight must be >= 0");
+    if (p->storage == TSDF_STORAGE_F16 && p->max_weight > 2048.0f)
+        return fail(TSDF_BAD_ARG, "fp16 weights are exact only up to 2048");
+
+    DeviceScope ds(p->device);
+    if (!ds.ok)
+        return fail(TSDF_HIP_ERROR, "hipSetDevice failed");
+    tsdf_grid* g = new tsdf_grid();
+    memset(g, 0, sizeof(*g));
+    g->p = *p;
+    g->vbl = (float)kVPE * vl;  // GPUVoxelGrid.cpp:15
+    for (int k = 0; k < kVPE; ++k)
+        g->off.v[k] = ((double)k + 0.5) * (double)vl;
+    hipDeviceProp_t prop;
+    int ncu = 256;
+    if (hipGetDeviceProperties(&prop, p->device) == hipSuccess)
+        ncu = prop.multiProcessorCount;
+    g->n_cu = ncu;
+
+    const size_t nb = p->n_buckets, nbr = p->n_bricks;
+    const size_t tsz = p->storage == TSDF_STORAGE_F16 ? 2 : 4;
+    tsdf_status st = TSDF_OK;
+    auto alloc = [&](void** ptr, size_t bytes) {
+        if (st != TSDF_OK)
+            return;
+        hipError_t e = hipMalloc(ptr, bytes);
+        if (e != hipSuccess)
+            st = fail(TSDF_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(e));
+    };
+    alloc((void**)&g->slots, nb * sizeof(unsigned long long));
+    alloc((void**)&g->slot_epoch, nb * sizeof(uint32_t));
+    alloc((void**)&g->coords, nbr * sizeof(int4));
+    alloc(&g->payload, nbr * 2 * kVPB * tsz);
+    if (p->with_color)
+        alloc((void**)&g->rgb, nbr * 3 * kVPB * sizeof(float));
+    alloc((void**)&g->touched, nb * sizeof(uint32_t));
+    alloc((void**)&g->ptmap, (size_t)p->height * p->width * sizeof(float4));
+    alloc((void**)&g->ctrl, sizeof(Ctrl));
+    if (st == TSDF_OK) {
+        hipError_t e = hipHostMalloc((void**)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
+        if (e != hipSuccess)
+            st = fail(TSDF_HIP_ERROR, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    if (st == TSDF_OK)
+        st = tsdf_reset(g, nullptr);
+    if (st == TSDF_OK && hipStreamSynchronize(nullptr) != hipSuccess)
+        st = fail(TSDF_HIP_ERROR, "synchronize after reset failed");
+    if (st != TSDF_OK) {
+        std::string keep = t_err;
+        tsdf_destroy(g);
+        t_err = keep;
+        return st;
+    }
+    *out = g;
+    return TSDF_OK;
+}
+
+void tsdf_destroy(tsdf_grid* g)
+{
+    if (!g)
+        return;
+    DeviceScope ds(g->p.device);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(g->slots);
+    (void)hipFree(g->slot_epoch);
+    (void)hipFree(g->coords);
+    (void)hipFree(g->payload);
+    (void)hipFree(g->rgb);
+    (void)hipFree(g->touched);
+    (void)hipFree(g->ptmap);
+    (void)hipFree(g->ctrl);
+    (void)hipFree(g->op_mask);
+    (void)hipFree(g->ptmap_batch);
+    (void)hipFree(g->ops_dev);
+    if (g->ctrl_host)
+        (void)hipHostFree(g->ctrl_host);
+    if (g->ops_host)
+        (void)hipHostFree(g->ops_host);
+    delete g;
+}
+
+tsdf_status tsdf_reset(tsdf_grid* g, void* stream)
+{
+    if (!g)
+        return fail(TSDF_BAD_ARG, "NULL grid");
+    DeviceScope ds(g->p.device);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t words = g->op_mask ? (size_t)g->p.n_buckets * kMaskWords : 0;
+    hipLaunchKernelGGL(k_reset, dim3(g->n_cu * 4), dim3(kThreads), 0, s, view(g), g->op_mask, words);
+    HIPC(hipGetLastError());
+    g->epoch = 0;
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_set_intrinsics(tsdf_grid* g, float fx, float fy, float cx, float cy)
+{
+    if (!g)
+        return fail(TSDF_BAD_ARG, "NULL grid");
+    if (!(std::isfinite(fx) && std::isfinite(fy) && std::isfinite(cx) && std::isfinite(cy)) || fx == 0.f ||
+        fy == 0.f)
+        return fail(TSDF_BAD_ARG, "intrinsics must be finite with fx, fy != 0");
+    g->cam.fx = fx;
+    g->cam.fy = fy;
+    g->cam.ci = (float)g->p.height - cy;  // GPUVoxelGrid.cpp:259
+    g->cam.cj = cx;                       // GPUVoxelGrid.cpp:260
+    g->intr_set = true;
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_integrate(tsdf_grid* g, const uint16_t* depth, int32_t pitch, const uint8_t* rgb,
+                           const float E[16], int32_t sign, void* stream)
+{
+    if (!g || !E)
+        return fail(TSDF_BAD_ARG, "NULL grid or pose");
+    if (!g->intr_set)
+        return fail(TSDF_NO_INTRINSICS, "tsdf_set_intrinsics was not called");
+    tsdf_status cs = check_image_args(g, depth, pitch);
+    if (cs != TSDF_OK)
+        return cs;
+    if (sign != 1 && sign != -1)
+        return fail(TSDF_BAD_ARG, "sign must be +1 or -1");
+    if (!pose_ok(E))
+        return fail(TSDF_BAD_POSE, "pose must be finite and rigid with last row 0 0 0 1");
+    DeviceScope ds(g->p.device);
+    hipStream_t s = (hipStream_t)stream;
+    OpDesc op;
+    op.depth = depth;
+    op.rgb = (g->p.with_color && rgb) ? rgb : nullptr;
+    op.ptmap = g->ptmap;
+    op.pitch = pitch;
+    op.sign = sign;
+    make_pose(E, &op.pose);
+    const uint32_t epoch = ++g->epoch;
+    const int par = (int)(epoch & 1u);
+    const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH);
+    hipLaunchKernelGGL(k_collect, cgrid, dim3(kThreads), 0, s, view(g), op, epoch, par);
+    const bool color = g->p.with_color != 0;
+    if (g->p.storage == TSDF_STORAGE_F16)
+        launch_integrate<__half>(g, op, par, s, color);
+    else
+        launch_integrate<float>(g, op, par, s, color);
+    HIPC(hipGetLastError());
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* const* depths, int32_t pitch,
+                                 const uint8_t* const* rgbs, const float* E, const int32_t* signs, void* stream)
+{
+    if (!g || !depths || !E || !signs || n_ops < 0)
+        return fail(TSDF_BAD_ARG, "NULL argument or negative op count");
+    if (!g->intr_set)
+        return fail(TSDF_NO_INTRINSICS, "tsdf_set_intrinsics was not called");
+    if (pitch < g->p.width)
+        return fail(TSDF_BAD_DIMS, "pitch smaller than the grid width");
+    for (int k = 0; k < n_ops; ++k) {
+        if (!depths[k])
+            return fail(TSDF_BAD_ARG, "depth pointer of an op is NULL");
+        if (signs[k] != 1 && signs[k] != -1)
+            return fail(TSDF_BAD_ARG, "sign must be +1 or -1");
+        if (!pose_ok(E + 16 * (size_t)k))
+            return fail(TSDF_BAD_POSE, "pose of an op is not a finite rigid transform");
+    }
+    if (n_ops == 0)
+        return TSDF_OK;
+    DeviceScope ds(g->p.device);
+    tsdf_status bs = ensure_batch_buffers(g);
+    if (bs != TSDF_OK)
+        return bs;
+    hipStream_t s = (hipStream_t)stream;
+    const bool color = g->p.with_color != 0;
+    const size_t frame = (size_t)g->p.height * g->p.width;
+    for (int base = 0; base < n_ops; base += kMaxOps) {
+        const int n = (n_ops - base) < kMaxOps ? (n_ops - base) : kMaxOps;
+        // the pinned staging buffer may still feed the previous chunk's copy
+        HIPC(hipStreamSynchronize(s));
+        for (int k = 0; k < n; ++k) {
+            OpDesc& op = g->ops_host[k];
+            op.depth = depths[base + k];
+            op.rgb = (color && rgbs) ? rgbs[base + k] : nullptr;
+            op.ptmap = g->ptmap_batch + (size_t)k * frame;
+            op.pitch = pitch;
+            op.sign = signs[base + k];
+            make_pose(E + 16 * (size_t)(base + k), &op.pose);
+        }
+        HIPC(hipMemcpyAsync(g->ops_dev, g->ops_host, n * sizeof(OpDesc), hipMemcpyHostToDevice, s));
+        const uint32_t epoch = ++g->epoch;
+        const int par = (int)(epoch & 1u);
+        const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH, n);
+        hipLaunchKernelGGL(k_collect_batch, cgrid, dim3(kThreads), 0, s, view(g), g->ops_dev, epoch, par,
+                           g->op_mask);
+        if (g->p.storage == TSDF_STORAGE_F16)
+            launch_integrate_batch<__half>(g, par, s, color);
+        else
+            launch_integrate_batch<float>(g, par, s, color);
+        HIPC(hipGetLastError());
+    }
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_reintegrate_batch(tsdf_grid* g, int32_t n, const uint16_t* const* depths, int32_t pitch,
+                                   const uint8_t* const* rgbs, const float* E_old, const float* E_new, void* stream)
+{
+    if (!g || !depths || !E_old || !E_new || n < 0)
+        return fail(TSDF_BAD_ARG, "NULL argument or negative keyframe count");
+    std::string err;
+    const uint16_t** d = new const uint16_t*[2 * (size_t)n + 1];
+    const uint8_t** c = rgbs ? new const uint8_t*[2 * (size_t)n + 1] : nullptr;
+    float* E = new float[32 * (size_t)n + 16];
+    int32_t* sg = new int32_t[2 * (size_t)n + 1];
+    for (int k = 0; k < n; ++k) {
+        d[2 * k] = d[2 * k + 1] = depths[k];
+        if (c)
+            c[2 * k] = c[2 * k + 1] = rgbs[k];
+        memcpy(E + 32 * (size_t)k, E_old + 16 * (size_t)k, 16 * sizeof(float));
+        memcpy(E + 32 * (size_t)k + 16, E_new + 16 * (size_t)k, 16 * sizeof(float));
+        sg[2 * k] = -1;
+        sg[2 * k + 1] = 1;
+    }
+    const tsdf_status r = tsdf_integrate_batch(g, 2 * n, d, pitch, c, E, sg, stream);
+    delete[] d;
+    delete[] c;
+    delete[] E;
+    delete[] sg;
+    return r;
+}
+
+tsdf_status tsdf_raycast(tsdf_grid* g, const float E[16], float* points, float* colors, float* depth,
+                         uint8_t* ub_flags, void* stream)
+{
+    if (!g || !E || !points)
+        return fail(TSDF_BAD_ARG, "NULL grid, pose or points");
+    if (!g->intr_set)
+        return fail(TSDF_NO_INTRINSICS, "tsdf_set_intrinsics was not called");
+    if (!pose_ok(E))
+        return fail(TSDF_BAD_POSE, "pose must be finite and rigid with last row 0 0 0 1");
+    DeviceScope ds(g->p.device);
+    hipStream_t s = (hipStream_t)stream;
+    Pose P;
+    make_pose(E, &P);
+    const int tiles_x = (g->p.width + 15) / 16, tiles_y = (g->p.height + 15) / 16;
+    const int n_tiles = tiles_x * tiles_y;
+    const int grid = ((n_tiles + 7) / 8) * 8;
+    const bool color = g->p.with_color != 0;
+    float* cols = color ? colors : nullptr;
+    if (g->p.storage == TSDF_STORAGE_F16) {
+        if (color)
+            hipLaunchKernelGGL((k_raycast<__half, true>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points, cols,
+                               depth, ub_flags, tiles_x, n_tiles);
+        else
+            hipLaunchKernelGGL((k_raycast<__half, false>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points,
+                               colors, depth, ub_flags, tiles_x, n_tiles);
+    } else {
+        if (color)
+            hipLaunchKernelGGL((k_raycast<float, true>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points, cols,
+                               depth, ub_flags, tiles_x, n_tiles);
+        else
+            hipLaunchKernelGGL((k_raycast<float, false>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points,
+                               colors, depth, ub_flags, tiles_x, n_tiles);
+    }
+    HIPC(hipGetLastError());
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_stats(tsdf_grid* g, tsdf_stats_out* out, void* stream)
+{
+    if (!g || !out)
+        return fail(TSDF_BAD_ARG, "NULL grid or output");
+    DeviceScope ds(g->p.device);
+    hipStream_t s = (hipStream_t)stream;
+    HIPC(hipMemcpyAsync(g->ctrl_host, g->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const Ctrl& c = *g->ctrl_host;
+    out->n_bricks = c.alloc_top < g->p.n_bricks ? c.alloc_top : g->p.n_bricks;
+    out->capacity = g->p.n_bricks;
+    out->overflow = c.overflow;
+    out->last_touched = g->epoch ? c.touched_count[g->epoch & 1u] : 0;
+    out->total_touched = c.total_touched;
+    out->total_new = c.total_new;
+    out->total_updated = c.total_updated;
+    out->total_valid_px = c.total_valid;
+    out->ray_hits = c.ray_hits;
+    out->ray_ub = c.ray_ub;
+    out->ray_samples = c.ray_samples;
+    out->ray_probes = c.ray_probes;
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_sync(tsdf_grid* g, void* stream)
+{
+    tsdf_stats_out st;
+    tsdf_status r = tsdf_stats(g, &st, stream);
+    if (r != TSDF_OK)
+        return r;
+    if (st.overflow)
+        return fail(TSDF_MAP_FULL, "brick pool or hash table overflowed; " + std::to_string(st.overflow) +
+                                       " allocations refused since the last reset");
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_export_bricks(tsdf_grid* g, int32_t* coords, float* sdf, float* w, float* rgb, uint64_t cap,
+                               uint64_t* n_out, void* stream)
+{
+    if (!g || !n_out)
+        return fail(TSDF_BAD_ARG, "NULL grid or count output");
+    tsdf_stats_out st;
+    tsdf_status r = tsdf_stats(g, &st, stream);
+    if (r != TSDF_OK)
+        return r;
+    *n_out = st.n_bricks;
+    const uint64_t n = st.n_bricks < cap ? st.n_bricks : cap;
+    if (n == 0)
+        return TSDF_OK;
+    DeviceScope ds(g->p.device);
+    hipStream_t s = (hipStream_t)stream;
+    const int grid = (int)(n < 65536 ? n : 65536);
+    float* c = g->p.with_color ? rgb : nullptr;
+    if (g->p.storage == TSDF_STORAGE_F16)
+        hipLaunchKernelGGL(k_export<__half>, dim3(grid), dim3(kThreads), 0, s, view(g), (unsigned)n, coords, sdf, w, c);
+    else
+        hipLaunchKernelGGL(k_export<float>, dim3(grid), dim3(kThreads), 0, s, view(g), (unsigned)n, coords, sdf, w, c);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s));
+    return TSDF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// tsdf_math.hpp -- scalar arithmetic of the TSDF path shared by every kernel.
+//
+// These functions define the floating-point contract that makes the GPU
+// results bit-identical to the reference algorithm: binary32 in source order
+// (the translation unit is compiled with -ffp-contract=off and correctly
+// rounded fp32 division / sqrt), the reference's double promotions where its
+// C++ source promotes, and the reference's (int) truncations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace tsdf {
+
+constexpr int kVPE = 8;    // voxelPerEdge, GPUVoxelGrid.h:6
+constexpr int kVPB = 512;  // voxelPerBlock, GPUVoxelGrid.h:8
+
+// Hash slot: high 32 bits = key (int32 reference hash as bits), low 32 bits =
+// brick id.  The all-ones word never occurs as an entry because brick ids stay
+// below 2^31.
+constexpr unsigned long long kSlotEmpty = ~0ull;
+constexpr uint32_t kIdPending = 0xFFFFFFFEu;   // key inserted, id being assigned
+constexpr uint32_t kIdOverflow = 0xFFFFFFFDu;  // key inserted, pool was full
+
+struct f3 {
+    float x, y, z;
+};
+
+// (int) of a float with x86-64 cvttss2si semantics: truncation toward zero,
+// INT_MIN for NaN / out of range (the GPU convert would saturate instead).
+__host__ __device__ __forceinline__ int f2i(float f)
+{
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : (int)0x80000000u;
+}
+
+// getPrimeHash, GPUVoxelGrid.h:124-128: 73856093x + 19349669y + 83492791z in
+// wrapping 32-bit int arithmetic.  The reference widens the int result to
+// unsigned long long by sign extension, so the 32-bit value is the whole key.
+__host__ __device__ __forceinline__ uint32_t prime_hash(int x, int y, int z)
+{
+    return (uint32_t)x * 73856093u + (uint32_t)y * 19349669u + (uint32_t)z * 83492791u;
+}
+
+// Bucket scatter for the open-addressing table (murmur3 finaliser).
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// getVBIndeces per axis, GPUVoxelGrid.h:162-190.
+__host__ __device__ __forceinline__ int vb_index(float a, float vbl)
+{
+    return (a >= 0) ? f2i(a / vbl) : f2i(a / vbl - 1.0f);
+}
+
+// getVinVBIndeces per axis, GPUVoxelGrid.h:192-221 (p - vbl * vbInd with
+// float * int -> float, kokkosFunctions.h:737-740).
+__host__ __device__ __forceinline__ int vin_index(float p, int vb, float vbl, float vl)
+{
+    float q = p - vbl * (float)vb;
+    return (q >= 0) ? f2i(q / vl) : f2i(q / vl - 1.0f);
+}
+
+// getVoxelMidPoint per axis, GPUVoxelGrid.h:275-278: the int * float product
+// is float, the (v + 0.5) * vl term and the sum are double, rounded to float.
+__host__ __device__ __forceinline__ float voxel_mid(int v, int vb, float vbl, float vl)
+{
+    return (float)((double)((float)vb * vbl) + ((double)v + 0.5) * (double)vl);
+}
+
+// Kvec3f scalar product, kokkosFunctions.h:212-214.
+__host__ __device__ __forceinline__ float dot3(f3 a, f3 b)
+{
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// transformPoint, GPUVoxelGrid.h:240-247, with the 3x4 row-major block.
+__host__ __device__ __forceinline__ f3 xform(f3 p, const float* E)
+{
+    f3 o;
+    o.x = p.x * E[0] + p.y * E[1] + p.z * E[2] + E[3];
+    o.y = p.x * E[4] + p.y * E[5] + p.z * E[6] + E[7];
+    o.z = p.x * E[8] + p.y * E[9] + p.z * E[10] + E[11];
+    return o;
+}
+
+// Camera model of the reference (getPoint GPUVoxelGrid.h:229-232 and
+// setIntrinsic GPUVoxelGrid.cpp:253-266): ci = H - cy, cj = cx.
+struct Camera {
+    float fx, fy, ci, cj;
+};
+
+__host__ __device__ __forceinline__ f3 get_point(const Camera& c, int i, int j, float d)
+{
+    f3 p;
+    p.x = ((float)j - c.cj) / c.fx * d;
+    p.y = ((float)i - c.ci) / c.fy * d;
+    p.z = d;
+    return p;
+}
+
+// Pose data of one call: E (camera->world) and E^-1 as 3x4 row-major blocks,
+// and Ez, the normalised third column of E.
+struct Pose {
+    float E[12];
+    float Ei[12];
+    float Ez[3];
+    float pad;
+};
+
+}  // namespace tsdf

@@ -1,0 +1,255 @@
+"""Host-side mirror of the reference class GPUVoxelGrid
+(src/VoxelGrid/GPUVoxelGrid.h:40-120) over the C ABI of include/tsdf.h.
+
+Method names, argument order and meaning follow the reference:
+``GPUVoxelGrid(c, trunc, voxlength, resi, resj)``, ``setIntrinsic``,
+``integrate(rgb, depth, extr, withColor)``, ``deIntegrate(...)``,
+``raycast(extr)`` filling the grid-owned ``rayPcd`` / ``rayColorPcd``.
+The reference's exec-space argument becomes the current torch stream.
+Differences: errors raise ``TsdfError`` instead of printing
+(GPUVoxelGrid.cpp:46-49) or corrupting memory (:497), and calls are
+asynchronous until ``sync()`` unless ``fence=True`` (the reference fences at
+the end of every call, GPUVoxelGrid.cpp:79,133,146).
+
+torch is used only for device memory and streams; every computation runs in
+libtsdf_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import TsdfParams, TsdfStats, check
+
+
+def _pose_ptr(E):
+    if isinstance(E, torch.Tensor):
+        E = E.detach().cpu().numpy()
+    a = np.ascontiguousarray(np.asarray(E, dtype=np.float32).reshape(16))
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _stream_ptr(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dev_ptr(t: torch.Tensor | None, dtypes, shape=None, name="tensor"):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype not in dtypes:
+        raise ValueError(f"{name} has dtype {t.dtype}, expected one of {dtypes}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise _lib.TsdfError(_lib.TSDF_BAD_DIMS, name,
+                             f"shape {tuple(t.shape)} does not match the grid {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+class GPUVoxelGrid:
+    """Voxel-hashed TSDF on one MI355X (see module docstring)."""
+
+    def __init__(self, c: int, trunc: float, voxlength: float, resi: int, resj: int, *,
+                 n_buckets: int | None = None, n_bricks: int | None = None,
+                 storage: str = "f32", withColor: bool = False, max_weight: float = 0.0,
+                 rmin: float = 0.2, d_max: float = 10.0, device: int | None = None):
+        L = _lib.load()
+        p = TsdfParams()
+        L.tsdf_default_params(C.byref(p))
+        p.voxel_length = voxlength
+        p.truncation = trunc
+        p.height = resi
+        p.width = resj
+        # reference capacity: c voxels -> c / 512 bricks (GPUVoxelGrid.cpp:11,27)
+        bricks = int(n_bricks) if n_bricks is not None else max(1, int(c) // 512)
+        if n_buckets is None:
+            n_buckets = 1 << max(10, (2 * bricks - 1).bit_length())
+        p.n_buckets = int(n_buckets)
+        p.n_bricks = bricks
+        p.storage = {"f32": _lib.STORAGE_F32, "f16": _lib.STORAGE_F16}[storage]
+        p.with_color = int(bool(withColor))
+        p.max_weight = max_weight
+        p.min_depth = rmin
+        p.max_ray_length = d_max
+        dev = torch.cuda.current_device() if device is None else int(device)
+        p.device = dev
+        self.params = p
+        self.device = torch.device("cuda", dev)
+        h = C.c_void_p()
+        check(L.tsdf_create(C.byref(p), C.byref(h)), "tsdf_create")
+        self._h = h
+        self._L = L
+        self.resi, self.resj = int(resi), int(resj)
+        self.d_trunc, self.vl = float(trunc), float(voxlength)
+        self.vbl = 8 * self.vl
+        self.d_max, self.rmin = float(d_max), float(rmin)
+        self.maxVoxelBlocks = bricks
+        self.withColor = bool(withColor)
+        self.intrinsicSet = False
+        # grid-owned raycast outputs (rayPcd, rayColorPcd, GPUVoxelGrid.h:67-69)
+        self.rayPcd = torch.zeros((resi, resj, 3), dtype=torch.float32, device=self.device)
+        self.rayColorPcd = torch.zeros((resi, resj, 3), dtype=torch.float32, device=self.device)
+        self.rayDepth = torch.zeros((resi, resj), dtype=torch.float32, device=self.device)
+        self.rayUB = torch.zeros((resi, resj), dtype=torch.uint8, device=self.device)
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.tsdf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, stream=None):
+        check(self._L.tsdf_reset(self._h, _stream_ptr(stream)), "tsdf_reset")
+
+    # -- reference surface -------------------------------------------------
+    def setIntrinsic(self, fx, fy=None, cx=None, cy=None):
+        """setIntrinsic (GPUVoxelGrid.cpp:253-266); accepts (fx, fy, cx, cy)
+        or a 3x3 intrinsic matrix."""
+        if fy is None:
+            K = np.asarray(fx, np.float64)
+            fx, fy, cx, cy = K[0, 0], K[1, 1], K[0, 2], K[1, 2]
+        check(self._L.tsdf_set_intrinsics(self._h, float(fx), float(fy), float(cx), float(cy)),
+              "tsdf_set_intrinsics")
+        self.fx, self.fy, self.cx, self.cy = float(fx), float(fy), float(cx), float(cy)
+        self.intrinsicSet = True
+
+    def _integrate(self, rgb, depth, extr, withColor, sign, stream, fence):
+        dptr = _dev_ptr(depth, (torch.uint16, torch.int16), None, "depth")
+        if depth.dim() != 2 or depth.shape[0] != self.resi or depth.shape[1] < self.resj:
+            raise _lib.TsdfError(_lib.TSDF_BAD_DIMS, "integrate",
+                                 "The input picture dimensions are not correct, they need to "
+                                 "match the constructor dimensions!")
+        cptr = None
+        if withColor and rgb is not None:
+            cptr = _dev_ptr(rgb, (torch.uint8,), (self.resi, self.resj, 3), "rgb")
+        keep, E = _pose_ptr(extr)
+        check(self._L.tsdf_integrate(self._h, dptr, int(depth.stride(0)), cptr, E, sign,
+                                     _stream_ptr(stream)), "tsdf_integrate")
+        if fence:
+            self.sync(stream)
+
+    def integrate(self, rgb, depth, extr, withColor=True, stream=None, fence=False):
+        """GPUVoxelGrid::integrate (GPUVoxelGrid.cpp:41-83)."""
+        self._integrate(rgb, depth, extr, withColor, 1, stream, fence)
+
+    def deIntegrate(self, rgb, depth, extr, withColor=True, stream=None, fence=False):
+        """GPUVoxelGrid::deIntegrate (GPUVoxelGrid.cpp:95-136)."""
+        self._integrate(rgb, depth, extr, withColor, -1, stream, fence)
+
+    def raycast(self, extr, stream=None, fence=False):
+        """GPUVoxelGrid::raycast (GPUVoxelGrid.cpp:139-147): fills rayPcd,
+        rayColorPcd (and rayDepth, rayUB)."""
+        keep, E = _pose_ptr(extr)
+        check(self._L.tsdf_raycast(self._h, E, C.c_void_p(self.rayPcd.data_ptr()),
+                                   C.c_void_p(self.rayColorPcd.data_ptr()),
+                                   C.c_void_p(self.rayDepth.data_ptr()),
+                                   C.c_void_p(self.rayUB.data_ptr()), _stream_ptr(stream)),
+              "tsdf_raycast")
+        if fence:
+            self.sync(stream)
+
+    # -- batched reintegration ----------------------------------------------
+    def integrate_batch(self, depths, poses, signs, rgbs=None, stream=None, fence=False):
+        """Ordered batch of integrate (+1) / deintegrate (-1) ops in one pass;
+        bit-identical to the same sequence of plain calls."""
+        n = len(depths)
+        if not (len(poses) == n and len(signs) == n):
+            raise ValueError("depths, poses and signs must have equal length")
+        pitch = None
+        dp = (C.c_void_p * max(1, n))()
+        for k, d in enumerate(depths):
+            dp[k] = _dev_ptr(d, (torch.uint16, torch.int16), (self.resi, d.shape[1]), "depth").value
+            pitch = int(d.stride(0)) if pitch is None else pitch
+            if int(d.stride(0)) != pitch:
+                raise ValueError("all depth images of a batch must share one pitch")
+        cp = None
+        if rgbs is not None:
+            cp = (C.c_void_p * max(1, n))()
+            for k, c in enumerate(rgbs):
+                cp[k] = _dev_ptr(c, (torch.uint8,), (self.resi, self.resj, 3), "rgb").value
+        E = np.ascontiguousarray(np.stack([np.asarray(p, np.float32).reshape(16) for p in poses]))
+        S = np.ascontiguousarray(np.asarray(signs, np.int32))
+        check(self._L.tsdf_integrate_batch(self._h, n, dp, pitch or self.resj, cp,
+                                           E.ctypes.data_as(C.POINTER(C.c_float)),
+                                           S.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           _stream_ptr(stream)), "tsdf_integrate_batch")
+        if fence:
+            self.sync(stream)
+
+    def reintegrate_batch(self, depths, old_poses, new_poses, rgbs=None, stream=None, fence=False):
+        """Keyframe k: deIntegrate(old_k) then integrate(new_k), k = 0..n-1
+        (the order of integrate.cpp:108-125), as one batch."""
+        n = len(depths)
+        dp = (C.c_void_p * max(1, n))()
+        pitch = None
+        for k, d in enumerate(depths):
+            dp[k] = _dev_ptr(d, (torch.uint16, torch.int16), (self.resi, d.shape[1]), "depth").value
+            pitch = int(d.stride(0)) if pitch is None else pitch
+        cp = None
+        if rgbs is not None:
+            cp = (C.c_void_p * max(1, n))()
+            for k, c in enumerate(rgbs):
+                cp[k] = _dev_ptr(c, (torch.uint8,), (self.resi, self.resj, 3), "rgb").value
+        Eo = np.ascontiguousarray(np.stack([np.asarray(p, np.float32).reshape(16) for p in old_poses]))
+        En = np.ascontiguousarray(np.stack([np.asarray(p, np.float32).reshape(16) for p in new_poses]))
+        check(self._L.tsdf_reintegrate_batch(self._h, n, dp, pitch or self.resj, cp,
+                                             Eo.ctypes.data_as(C.POINTER(C.c_float)),
+                                             En.ctypes.data_as(C.POINTER(C.c_float)),
+                                             _stream_ptr(stream)), "tsdf_reintegrate_batch")
+        if fence:
+            self.sync(stream)
+
+    # -- state -------------------------------------------------------------
+    def sync(self, stream=None):
+        check(self._L.tsdf_sync(self._h, _stream_ptr(stream)), "tsdf_sync")
+
+    def stats(self, stream=None) -> dict:
+        s = TsdfStats()
+        check(self._L.tsdf_stats(self._h, C.byref(s), _stream_ptr(stream)), "tsdf_stats")
+        return s.as_dict()
+
+    def size(self) -> int:
+        """voxelMap2.size()"""
+        return self.stats()["n_bricks"]
+
+    def export_bricks(self, stream=None) -> dict:
+        n = self.size()
+        dev = self.device
+        coords = torch.empty((max(n, 1), 3), dtype=torch.int32, device=dev)
+        sdf = torch.empty((max(n, 1), 512), dtype=torch.float32, device=dev)
+        w = torch.empty((max(n, 1), 512), dtype=torch.float32, device=dev)
+        rgb = torch.empty((max(n, 1), 512, 3), dtype=torch.float32, device=dev) if self.withColor else None
+        nout = C.c_uint64(0)
+        check(self._L.tsdf_export_bricks(self._h, C.c_void_p(coords.data_ptr()), C.c_void_p(sdf.data_ptr()),
+                                         C.c_void_p(w.data_ptr()),
+                                         C.c_void_p(rgb.data_ptr()) if rgb is not None else None,
+                                         n, C.byref(nout), _stream_ptr(stream)), "tsdf_export_bricks")
+        m = min(int(nout.value), n)
+        out = {"coords": coords[:m], "sdf": sdf[:m], "w": w[:m]}
+        if rgb is not None:
+            out["rgb"] = rgb[:m]
+        return out
+
+
+def pose_prepare(E):
+    """(E^-1, Ez) exactly as every kernel receives them (host helper)."""
+    L = _lib.load()
+    a = np.ascontiguousarray(np.asarray(E, np.float32).reshape(16))
+    Ei = np.zeros(16, np.float32)
+    Ez = np.zeros(3, np.float32)
+    check(L.tsdf_pose_prepare(a.ctypes.data_as(C.POINTER(C.c_float)),
+                              Ei.ctypes.data_as(C.POINTER(C.c_float)),
+                              Ez.ctypes.data_as(C.POINTER(C.c_float))), "tsdf_pose_prepare")
+    return Ei.reshape(4, 4), Ez
