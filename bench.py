@@ -1,0 +1,335 @@
+#!/usr/bin/env python3
+"""TSDF integrate + raycast throughput on one MI355X.
+
+Workload ("room_large"): 600 synthetic 848x480 uint16 depth frames of a seeded
+random room (floor/walls, 6 boxes, 2 spheres) along a random walk of 5 mm /
+0.5 deg per step; voxel 4 mm, truncation 20 mm (5 voxels), 2^22 hash buckets,
+max weight 128, fp32 sdf + fp32 weight.  One step is one pass over the whole
+sequence from an empty map: for pose i, integrate frame i; when i >= 10 and
+i % 10 == 0 deintegrate frame i - 10 at its pose; raycast pose i (world points
++ depth).  value = 600 * 848 * 480 / step time, in Mpx/s.  Depth frames are
+resident in HBM before the timed region starts.
+
+Besides the headline line the JSON carries:
+  roofline      the dominant kernel's algorithmic bytes per launch / its
+                average duration (HIP events around every launch on its
+                stream, over a second timed pass), against 8 TB/s
+  kernels       the same figures for every kernel of the path
+  cpu_baseline  the oracle (C restatement of the reference) on the first
+                frames of the same sequence, on the host cores
+  batched_reintegration
+                64 keyframes re-fused at perturbed poses (valid fraction
+                U(0.3, 1) per keyframe): one batched pass vs 128 plain calls,
+                with the resulting TSDFs compared for identity
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+S_V = 8        # bytes per voxel: fp32 sdf + fp32 weight
+S_H = 8        # bytes per touched hash slot
+S_OUT = 16     # raycast output per pixel: world point (12) + depth (4)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=600)
+    ap.add_argument("--cpu-frames", type=int, default=24,
+                    help="frames of the sequence the CPU baseline processes")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-reintegration", action="store_true")
+    ap.add_argument("--keyframes", type=int, default=64)
+    ap.add_argument("--footprint-samples", type=int, default=12)
+    return ap.parse_args()
+
+
+def workload(frames):
+    from superscanner_amd import synth
+    cfg = dict(synth.ROOM_LARGE)
+    poses64 = synth.traj_random_walk(n=frames, seed=2)
+    prims = synth.scene_room(seed=1)
+    t0 = time.time()
+    depth = synth.frames(prims, poses64, cfg, seed=7)
+    gen_s = time.time() - t0
+    poses = np.stack([synth.pose_f32(T) for T in poses64])
+    return cfg, prims, poses64, poses, depth, gen_s
+
+
+def schedule(frames):
+    """(op, frame) list of one step: ('i', k) integrate, ('d', k) deintegrate,
+    ('r', k) raycast."""
+    ops = []
+    for i in range(frames):
+        ops.append(("i", i))
+        if i >= 10 and i % 10 == 0:
+            ops.append(("d", i - 10))
+        ops.append(("r", i))
+    return ops
+
+
+class Runner:
+    """Issues one step straight through the C ABI with prebuilt arguments."""
+
+    def __init__(self, grid, depth_dev, poses, frames):
+        import torch
+        self.g = grid
+        self.L = grid._L
+        self.h = grid._h
+        self.frames = frames
+        H, W = grid.resi, grid.resj
+        self.pitch = W
+        self.base = depth_dev.data_ptr()
+        self.fsz = H * W * 2
+        self.E = np.ascontiguousarray(poses.reshape(-1, 16).astype(np.float32))
+        self.Ep = [self.E[k].ctypes.data_as(C.POINTER(C.c_float)) for k in range(len(self.E))]
+        self.pts = torch.empty((H, W, 3), dtype=torch.float32, device=depth_dev.device)
+        self.dep = torch.empty((H, W), dtype=torch.float32, device=depth_dev.device)
+        self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.ops = schedule(frames)
+
+    def step(self):
+        L, h, s = self.L, self.h, self.stream
+        rc = L.tsdf_reset(h, s)
+        pts, dep = C.c_void_p(self.pts.data_ptr()), C.c_void_p(self.dep.data_ptr())
+        for op, k in self.ops:
+            if op == "r":
+                rc |= L.tsdf_raycast(h, self.Ep[k], pts, None, dep, None, s)
+            else:
+                rc |= L.tsdf_integrate(h, C.c_void_p(self.base + k * self.fsz), self.pitch, None, self.Ep[k],
+                                       1 if op == "i" else -1, s)
+        if rc:
+            raise RuntimeError("a TSDF call failed: " + L.tsdf_last_error().decode())
+
+
+def footprint_pass(runner, n_samples):
+    """Re-runs the sequence untimed and measures the raycast footprint
+    (distinct voxels / keys) at n_samples evenly spaced poses."""
+    g = runner.g
+    L, h, s = runner.L, runner.h, runner.stream
+    want = set(np.linspace(0, runner.frames - 1, n_samples).astype(int).tolist())
+    L.tsdf_reset(h, s)
+    u_vox, u_blk = [], []
+    for op, k in runner.ops:
+        if op == "r":
+            if k in want:
+                fp = g.raycast_footprint(runner.E[k].reshape(4, 4))
+                u_vox.append(fp["u_vox"])
+                u_blk.append(fp["u_blk"])
+        else:
+            L.tsdf_integrate(h, C.c_void_p(runner.base + k * runner.fsz), runner.pitch, None, runner.Ep[k],
+                             1 if op == "i" else -1, s)
+    g.sync()
+    return float(np.mean(u_vox)), float(np.mean(u_blk))
+
+
+def cpu_baseline(cfg, poses, depth, n):
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    oracle.set_threads(threads)
+    o = oracle.OracleGrid(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], max_bricks=1 << 22,
+                          max_weight=cfg["max_weight"])
+    o.set_intrinsics(*cfg["intr"])
+    ops = schedule(n)
+    t0 = time.perf_counter()
+    for op, k in ops:
+        if op == "r":
+            o.raycast(poses[k])
+        else:
+            o.integrate(depth[k], poses[k], 1 if op == "i" else -1)
+    t = time.perf_counter() - t0
+    o.close()
+    return {"value": round(n * cfg["H"] * cfg["W"] / t / 1e6, 3), "unit": "Mpx/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {n} of the 600 poses of the same sequence (integrate, deintegrate every 10th, "
+                      f"raycast each pose), {t:.1f} s wall on {threads} OpenMP threads"}
+
+
+def reintegration(cfg, prims, poses64, keyframes, dev):
+    """Batched vs plain re-fusion of `keyframes` keyframes."""
+    import torch
+    from superscanner_amd import synth
+    from superscanner_amd.voxel_grid import GPUVoxelGrid
+    rng = np.random.default_rng(21)
+    n = len(poses64)
+    kf = np.linspace(0, n - 1, keyframes).astype(int)
+    keep = rng.uniform(0.3, 1.0, keyframes)
+    kdepth = synth.frames(prims, [poses64[k] for k in kf], cfg, seed=31, keep=keep)
+    old = np.stack([synth.pose_f32(poses64[k]) for k in kf])
+    new = np.stack([synth.pose_f32(synth.perturb(poses64[k], rng)) for k in kf])
+    kd = torch.from_numpy(kdepth.view(np.int16)).to(dev)
+    H, W = cfg["H"], cfg["W"]
+
+    def fresh():
+        g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
+                         n_bricks=1 << 21, max_weight=cfg["max_weight"])
+        g.setIntrinsic(*cfg["intr"])
+        for k in range(keyframes):
+            g.integrate(None, kd[k], old[k])
+        g.sync()
+        return g
+
+    a, b = fresh(), fresh()
+    torch.cuda.synchronize()
+    # plain: the integrate.cpp:108-125 loop, one deIntegrate + integrate per keyframe
+    t0 = time.perf_counter()
+    for k in range(keyframes):
+        a.deIntegrate(None, kd[k], old[k])
+        a.integrate(None, kd[k], new[k])
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    # plain with the reference's fence after every call (GPUVoxelGrid.cpp:79,133)
+    c = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(keyframes):
+        c.deIntegrate(None, kd[k], old[k], fence=True)
+        c.integrate(None, kd[k], new[k], fence=True)
+    t_fenced = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.reintegrate_batch([kd[k] for k in range(keyframes)], old, new)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    ea, eb = a.export_bricks(), b.export_bricks()
+    oa = torch.argsort(ea["coords"][:, 0] * 4194304 + ea["coords"][:, 1] * 2048 + ea["coords"][:, 2])
+    ob = torch.argsort(eb["coords"][:, 0] * 4194304 + eb["coords"][:, 1] * 2048 + eb["coords"][:, 2])
+    same = bool(torch.equal(ea["coords"][oa], eb["coords"][ob]) and
+                torch.equal(ea["sdf"][oa].view(torch.int32), eb["sdf"][ob].view(torch.int32)) and
+                torch.equal(ea["w"][oa].view(torch.int32), eb["w"][ob].view(torch.int32)))
+    for g in (a, b, c):
+        g.close()
+    return {"keyframes": keyframes, "plain_calls": 2 * keyframes, "ms_plain": round(t_plain * 1e3, 3),
+            "ms_plain_fenced": round(t_fenced * 1e3, 3), "ms_batched": round(t_batch * 1e3, 3),
+            "speedup_vs_plain": round(t_plain / t_batch, 2), "speedup_vs_fenced": round(t_fenced / t_batch, 2),
+            "bit_identical": same}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build has no "
+              "multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from superscanner_amd.voxel_grid import GPUVoxelGrid
+    assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    dev = torch.device("cuda", 0)
+    frames = args.frames
+    cfg, prims, poses64, poses, depth, gen_s = workload(frames)
+    H, W = cfg["H"], cfg["W"]
+    print(f"[bench] generated {frames} frames {W}x{H} in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    depth_dev = torch.from_numpy(depth.view(np.int16)).to(dev)
+    grid = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
+                        n_bricks=1 << 21, max_weight=cfg["max_weight"])
+    grid.setIntrinsic(*cfg["intr"])
+    run = Runner(grid, depth_dev, poses, frames)
+
+    for _ in range(args.warmup):
+        run.step()
+    torch.cuda.synchronize()
+    st = grid.stats()  # counters of one step (each step starts from reset)
+    print(f"[bench] warmup done: {st['n_bricks']} bricks", file=sys.stderr, flush=True)
+
+    # timed region
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        run.step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    print(f"[bench] timed {args.steps} steps: {ms:.2f} ms/step (wall {wall * 1e3 / args.steps:.2f})",
+          file=sys.stderr, flush=True)
+
+    # second timed pass with an event pair around every launch
+    grid.profile(True)
+    grid.profile_read(reset=True)
+    torch.cuda.synchronize()
+    p0 = time.perf_counter()
+    for _ in range(args.steps):
+        run.step()
+    torch.cuda.synchronize()
+    ms_prof = (time.perf_counter() - p0) * 1e3 / args.steps
+    kt = grid.profile_read(reset=True)
+    grid.profile(False)
+
+    u_vox, u_blk = footprint_pass(run, args.footprint_samples)
+    n_int = sum(1 for o, _ in run.ops if o in "id")
+    n_ray = sum(1 for o, _ in run.ops if o == "r")
+    px = H * W
+    # algorithmic bytes per launch, from the step's own counters
+    touched = st["total_touched"] / n_int
+    upd = st["total_updated"] / n_int
+    new = st["total_new"] / n_int
+    valid = st["total_valid_px"] / n_int
+    bytes_per = {
+        "collect": 2 * px + (touched + new) * S_H,
+        "integrate": touched * 512 * S_V + upd * S_V + touched * S_H,
+        "raycast": u_vox * S_V + u_blk * S_H + px * S_OUT,
+    }
+    kernels = {}
+    for name in ("collect", "integrate", "raycast"):
+        k = kt[name]
+        if k["launches"] == 0:
+            continue
+        avg_ms = k["ms"] / k["launches"]
+        gbs = bytes_per[name] / (avg_ms * 1e-3) / 1e9
+        kernels[name] = {"avg_us": round(avg_ms * 1e3, 2), "launches_per_step": k["launches"] // args.steps,
+                         "bytes_per_launch": int(bytes_per[name]), "achieved_GBs": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "share_of_step": round(k["ms"] / args.steps / ms_prof, 3)}
+    dom = max(kernels, key=lambda n: kt[n]["ms"])
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": kernels[dom]["frac"], "traffic": traffic,
+            "bytes_per_launch": kernels[dom]["bytes_per_launch"], "avg_us": kernels[dom]["avg_us"]}
+
+    value = frames * px / (ms * 1e-3) / 1e6
+    out = {
+        "metric": "TSDF integrate+raycast Mpx/s, % HBM roofline; batched vs plain reintegrate",
+        "value": round(value, 2), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "room_large: 600 frames 848x480 uint16 depth, random room + random walk, "
+                               "vl 4 mm, trunc 20 mm, 2^22 buckets, wmax 128; integrate each frame, "
+                               "deintegrate every 10th, raycast each pose",
+                   "frames": frames, "height": H, "width": W, "voxel_length": cfg["voxel_length"],
+                   "truncation": cfg["truncation"], "n_buckets": cfg["n_buckets"],
+                   "integrate_calls": n_int, "raycast_calls": n_ray, "bricks": st["n_bricks"]},
+        "roofline": roof,
+        "kernels": kernels,
+        "ms_per_step_profiled": round(ms_prof, 3),
+        "raycast_footprint": {"u_vox": round(u_vox), "u_blk": round(u_blk), "samples": args.footprint_samples},
+    }
+    if not args.no_reintegration:
+        out["batched_reintegration"] = reintegration(cfg, prims, poses64, args.keyframes, dev)
+        print("[bench] reintegration done", file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, poses, depth, min(args.cpu_frames, frames))
+        print("[bench] cpu baseline done", file=sys.stderr, flush=True)
+    grid.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -86,6 +86,23 @@ typedef struct {
     uint64_t ray_probes;    /* raycast hash lookups since reset                      */
 } tsdf_stats_out;
 
+/* Kernel kinds timed by tsdf_profile (no reference counterpart: the
+ * reference names its kernels for Kokkos-Tools, kernelFilters.lst). */
+enum {
+    TSDF_K_RESET = 0,           /* table / counter reset                         */
+    TSDF_K_COLLECT = 1,         /* block allocation ("Integration Collection")   */
+    TSDF_K_INTEGRATE = 2,       /* brick update ("Integration Core Kernel")      */
+    TSDF_K_COLLECT_BATCH = 3,   /* allocation for a batch of ops                 */
+    TSDF_K_INTEGRATE_BATCH = 4, /* brick update for a batch of ops               */
+    TSDF_K_RAYCAST = 5,         /* "Raycasting Kernel"                           */
+    TSDF_N_KERNEL_KINDS = 6
+};
+
+typedef struct {
+    double ms[TSDF_N_KERNEL_KINDS];         /* summed device time per kind   */
+    uint64_t launches[TSDF_N_KERNEL_KINDS]; /* launches timed per kind       */
+} tsdf_kernel_times;
+
 /* Fills the defaults of the reference: d_max 10 m, rmin 0.2 m, no weight
  * cap, fp32 storage, no colour (GPUVoxelGrid.h:55-62). */
 void tsdf_default_params(tsdf_params *p);
@@ -140,6 +157,21 @@ tsdf_status tsdf_reintegrate_batch(tsdf_grid *g, int32_t n, const uint16_t *cons
  * point or NULL (extension).  ub_flags: device [H][W] uint8 or NULL. */
 tsdf_status tsdf_raycast(tsdf_grid *g, const float E_cam2world[16], float *points,
                          float *colors, float *depth, uint8_t *ub_flags, void *stream);
+
+/* Diagnostic raycast from E that counts the distinct voxels read and the
+ * distinct block keys probed (hits and misses) by the march of
+ * tsdf_raycast: the compulsory-traffic terms of the raycast's roofline.
+ * Synchronous; writes no image output. */
+tsdf_status tsdf_raycast_footprint(tsdf_grid *g, const float E_cam2world[16],
+                                   uint64_t *distinct_voxels, uint64_t *distinct_keys,
+                                   void *stream);
+
+/* Per-kernel timing: while enabled, every launch is bracketed by a pair of
+ * HIP events on its stream.  tsdf_profile_read waits for the recorded events
+ * and returns the summed device time per kernel kind (and clears the sums
+ * when reset != 0). */
+tsdf_status tsdf_profile(tsdf_grid *g, int32_t enable);
+tsdf_status tsdf_profile_read(tsdf_grid *g, tsdf_kernel_times *out, int32_t reset);
 
 /* Waits for `stream`, then reports TSDF_MAP_FULL if any allocation was
  * refused since the last reset. */

@@ -32,8 +32,10 @@ EXPORTS = (
     "tsdf_set_intrinsics", "tsdf_integrate", "tsdf_integrate_batch",
     "tsdf_reintegrate_batch", "tsdf_raycast", "tsdf_sync", "tsdf_stats",
     "tsdf_export_bricks", "tsdf_pose_prepare", "tsdf_status_string",
-    "tsdf_last_error",
+    "tsdf_last_error", "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
 )
+
+KERNEL_KINDS = ("reset", "collect", "integrate", "collect_batch", "integrate_batch", "raycast")
 
 
 class TsdfParams(C.Structure):
@@ -61,6 +63,14 @@ class TsdfStats(C.Structure):
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class TsdfKernelTimes(C.Structure):
+    _fields_ = [("ms", C.c_double * 6), ("launches", C.c_uint64 * 6)]
+
+    def as_dict(self):
+        return {k: {"ms": float(self.ms[i]), "launches": int(self.launches[i])}
+                for i, k in enumerate(KERNEL_KINDS)}
 
 
 class TsdfError(RuntimeError):
@@ -100,13 +110,17 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_stats.argtypes = [vp, P(TsdfStats), vp]
     L.tsdf_export_bricks.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, P(C.c_uint64), vp]
     L.tsdf_pose_prepare.argtypes = [fp, fp, fp]
+    L.tsdf_raycast_footprint.argtypes = [vp, fp, P(C.c_uint64), P(C.c_uint64), vp]
+    L.tsdf_profile.argtypes = [vp, C.c_int32]
+    L.tsdf_profile_read.argtypes = [vp, P(TsdfKernelTimes), C.c_int32]
     L.tsdf_status_string.argtypes = [C.c_int]
     L.tsdf_status_string.restype = C.c_char_p
     L.tsdf_last_error.argtypes = []
     L.tsdf_last_error.restype = C.c_char_p
     for name in ("tsdf_create", "tsdf_reset", "tsdf_set_intrinsics", "tsdf_integrate",
                  "tsdf_integrate_batch", "tsdf_reintegrate_batch", "tsdf_raycast",
-                 "tsdf_sync", "tsdf_stats", "tsdf_export_bricks", "tsdf_pose_prepare"):
+                 "tsdf_sync", "tsdf_stats", "tsdf_export_bricks", "tsdf_pose_prepare",
+                 "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

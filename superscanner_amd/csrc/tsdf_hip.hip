@@ -597,14 +597,65 @@ __device__ __forceinline__ bool vin_ok(int x, int y, int z)
     return (unsigned)x < (unsigned)kVPE && (unsigned)y < (unsigned)kVPE && (unsigned)z < (unsigned)kVPE;
 }
 
+// Distinct voxels / keys a raycast reads (the compulsory-bytes terms of its
+// roofline); only the k_raycast<.., kStats = true> instantiation touches it.
+struct Footprint {
+    uint32_t* vox_bits;         // [n_bricks][16] one bit per voxel
+    unsigned long long* keys;   // open-addressing set of probed keys
+    uint32_t key_mask;
+    unsigned long long* counts; // [0] distinct keys, [1] set overflow, [2] distinct voxels
+};
+
+__device__ void fp_key(const Footprint& fp, uint32_t key)
+{
+    uint32_t b = mix32(key) & fp.key_mask;
+    for (int p = 0; p < 256; ++p) {
+        const unsigned long long s = fp.keys[b];
+        if (s == (unsigned long long)key)
+            return;
+        if (s == kSlotEmpty) {
+            const unsigned long long old = atomicCAS(&fp.keys[b], kSlotEmpty, (unsigned long long)key);
+            if (old == kSlotEmpty) {
+                atomicAdd(&fp.counts[0], 1ull);
+                return;
+            }
+            if (old == (unsigned long long)key)
+                return;
+        }
+        b = (b + 1) & fp.key_mask;
+    }
+    atomicAdd(&fp.counts[1], 1ull);
+}
+
+template <bool kStats>
+__device__ __forceinline__ int probe(const GridView& g, const Footprint& fp, int x, int y, int z,
+                                     unsigned int& probes)
+{
+    ++probes;
+    if (kStats)
+        fp_key(fp, prime_hash(x, y, z));
+    return find_brick(g, x, y, z);
+}
+
+template <bool kStats>
+__device__ __forceinline__ void note_voxel(const Footprint& fp, int id, int vx, int vy, int vz,
+                                           unsigned int& samples)
+{
+    ++samples;
+    if (kStats) {
+        const int v = vx + kVPE * vy + kVPE * kVPE * vz;
+        atomicOr(&fp.vox_bits[(size_t)id * 16 + (v >> 5)], 1u << (v & 31));
+    }
+}
+
 struct RayOut {
     f3 p, c;
     int code;  // 1 hit, 0 miss, -flag UB
     unsigned int samples, probes;
 };
 
-template <typename T, bool kColor>
-__device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
+template <typename T, bool kColor, bool kStats>
+__device__ RayOut march(const GridView& g, const Pose& pose, int i, int j, const Footprint& fp)
 {
     RayOut out;
     out.p = {0.f, 0.f, 0.f};
@@ -631,8 +682,7 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
     int ox = 0, oy = 0, oz = 0;
     while (!done && !Fn && l < dmax) {
         int bx = vb_index(rv.x, vbl), by = vb_index(rv.y, vbl), bz = vb_index(rv.z, vbl);
-        int id = find_brick(g, bx, by, bz);
-        ++out.probes;
+        int id = probe<kStats>(g, fp, bx, by, bz, out.probes);
         int count = 0;
         while (id < 0 && l < dmax) {
             rv.x += vbl * dir.x;
@@ -642,8 +692,7 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
             bx = vb_index(rv.x, vbl);
             by = vb_index(rv.y, vbl);
             bz = vb_index(rv.z, vbl);
-            id = find_brick(g, bx, by, bz);
-            ++out.probes;
+            id = probe<kStats>(g, fp, bx, by, bz, out.probes);
             ++count;
         }
         if (l >= dmax)
@@ -657,8 +706,7 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
             bx = vb_index(rv.x, vbl);
             by = vb_index(rv.y, vbl);
             bz = vb_index(rv.z, vbl);
-            id = find_brick(g, bx, by, bz);
-            ++out.probes;
+            id = probe<kStats>(g, fp, bx, by, bz, out.probes);
             while (id < 0 && l < dmax) {
                 rv.x += dt * dir.x;
                 rv.y += dt * dir.y;
@@ -667,8 +715,7 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
                 bx = vb_index(rv.x, vbl);
                 by = vb_index(rv.y, vbl);
                 bz = vb_index(rv.z, vbl);
-                id = find_brick(g, bx, by, bz);
-                ++out.probes;
+                id = probe<kStats>(g, fp, bx, by, bz, out.probes);
             }
         }
         if (l >= dmax)
@@ -684,7 +731,7 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
                 return out;
             }
             const Vox<T> cv = read_voxel<T>(g, cur, vx, vy, vz);
-            ++out.samples;
+            note_voxel<kStats>(fp, cur, vx, vy, vz, out.samples);
             if (cv.w > 0) {
                 if (cv.s > 0) {
                     Lp = true;
@@ -720,8 +767,7 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
                                        voxel_mid(vz, bz, vbl, vl)};
                     int ob = cur;
                     if (bx != ox || by != oy || bz != oz) {
-                        ob = find_brick(g, bx, by, bz);
-                        ++out.probes;
+                        ob = probe<kStats>(g, fp, bx, by, bz, out.probes);
                         if (ob < 0) {
                             out.code = -TSDF_RAY_UB_MISSING_OUT;
                             return out;
@@ -732,7 +778,7 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
                         return out;
                     }
                     const Vox<T> vo = read_voxel<T>(g, ob, vx, vy, vz);
-                    ++out.samples;
+                    note_voxel<kStats>(fp, ob, vx, vy, vz, out.samples);
                     const float xx = (vo.w * vo.s / vl + cv.w * (1.0f - cv.s / vl)) / (cv.w + vo.w);
                     f3 p;
                     p.x = vm_out.x + xx * (vm_in.x - vm_out.x);
@@ -777,9 +823,10 @@ __device__ RayOut march(const GridView& g, const Pose& pose, int i, int j)
     return out;
 }
 
-template <typename T, bool kColor>
+template <typename T, bool kColor, bool kStats>
 __global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, float* points, float* colors,
-                                                      float* depth, uint8_t* ub, int tiles_x, int n_tiles)
+                                                      float* depth, uint8_t* ub, int tiles_x, int n_tiles,
+                                                      Footprint fp)
 {
     __shared__ unsigned int s_cnt[4];
     if (threadIdx.x < 4)
@@ -793,7 +840,7 @@ __global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, flo
     const int i = (tile / tiles_x) * 16 + (wv >> 1) * 8 + (lane >> 3);
     const int j = (tile % tiles_x) * 16 + (wv & 1) * 8 + (lane & 7);
     if (tile < n_tiles && i < g.H && j < g.W) {
-        const RayOut r = march<T, kColor>(g, pose, i, j);
+        const RayOut r = march<T, kColor, kStats>(g, pose, i, j, fp);
         const size_t o = (size_t)i * g.W + j;
         f3 p = r.p, c = r.c;
         if (r.code < 0) {
@@ -826,6 +873,16 @@ __global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, flo
         if (s_cnt[2]) atomicAdd(&g.ctrl->ray_samples, (unsigned long long)s_cnt[2]);
         if (s_cnt[3]) atomicAdd(&g.ctrl->ray_probes, (unsigned long long)s_cnt[3]);
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_popcount(const uint32_t* bits, size_t n, unsigned long long* out)
+{
+    unsigned int c = 0;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        c += __popc(bits[k]);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c)
+        atomicAdd(out, (unsigned long long)c);
 }
 
 // ---------------------------------------------------------------------------
@@ -883,6 +940,19 @@ struct tsdf_grid {
     float4* ptmap_batch;
     OpDesc* ops_dev;
     OpDesc* ops_host;  // pinned
+    // raycast footprint buffers (allocated on first tsdf_raycast_footprint)
+    uint32_t* fp_bits;
+    unsigned long long* fp_keys;
+    unsigned long long* fp_counts;
+    float* fp_points;
+    uint32_t fp_key_mask;
+    // per-kernel event timing (tsdf_profile)
+    bool prof_on;
+    int prof_used, prof_cap;
+    hipEvent_t* prof_ev;  // 2 per record
+    int* prof_kind;
+    double prof_ms[TSDF_N_KERNEL_KINDS];
+    uint64_t prof_n[TSDF_N_KERNEL_KINDS];
 };
 
 namespace {
@@ -1027,6 +1097,82 @@ tsdf_status ensure_batch_buffers(tsdf_grid* g)
     HIPC(hipHostMalloc(&g->ops_host, kMaxOps * sizeof(OpDesc), hipHostMallocDefault));
     HIPC(hipDeviceSynchronize());
     return TSDF_OK;
+}
+
+// Per-kernel timing: an event pair around each launch on its own stream.
+tsdf_status prof_flush(tsdf_grid* g)
+{
+    if (g->prof_used == 0)
+        return TSDF_OK;
+    HIPC(hipEventSynchronize(g->prof_ev[2 * g->prof_used - 1]));
+    for (int k = 0; k < g->prof_used; ++k) {
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, g->prof_ev[2 * k], g->prof_ev[2 * k + 1]));
+        g->prof_ms[g->prof_kind[k]] += ms;
+        g->prof_n[g->prof_kind[k]] += 1;
+    }
+    g->prof_used = 0;
+    return TSDF_OK;
+}
+
+struct ProfScope {
+    tsdf_grid* g;
+    hipStream_t s;
+    int slot = -1;
+    ProfScope(tsdf_grid* g_, hipStream_t s_, int kind) : g(g_), s(s_)
+    {
+        if (!g->prof_on)
+            return;
+        if (g->prof_used == g->prof_cap) {
+            const int nc = g->prof_cap ? 2 * g->prof_cap : 1024;
+            hipEvent_t* ev = (hipEvent_t*)realloc(g->prof_ev, 2 * (size_t)nc * sizeof(hipEvent_t));
+            int* kd = (int*)realloc(g->prof_kind, (size_t)nc * sizeof(int));
+            if (!ev || !kd) {
+                if (ev) g->prof_ev = ev;
+                if (kd) g->prof_kind = kd;
+                return;
+            }
+            g->prof_ev = ev;
+            g->prof_kind = kd;
+            for (int k = 2 * g->prof_cap; k < 2 * nc; ++k)
+                if (hipEventCreate(&g->prof_ev[k]) != hipSuccess)
+                    return;
+            g->prof_cap = nc;
+        }
+        slot = g->prof_used++;
+        g->prof_kind[slot] = kind;
+        (void)hipEventRecord(g->prof_ev[2 * slot], s);
+    }
+    ~ProfScope()
+    {
+        if (slot >= 0)
+            (void)hipEventRecord(g->prof_ev[2 * slot + 1], s);
+    }
+};
+
+template <bool kStats>
+void launch_raycast(const tsdf_grid* g, const Pose& P, float* points, float* colors, float* depth, uint8_t* ub,
+                    const Footprint& fp, hipStream_t s)
+{
+    const int tiles_x = (g->p.width + 15) / 16, tiles_y = (g->p.height + 15) / 16;
+    const int n_tiles = tiles_x * tiles_y;
+    const dim3 grid(((n_tiles + 7) / 8) * 8);
+    const bool color = g->p.with_color != 0;
+    if (g->p.storage == TSDF_STORAGE_F16) {
+        if (color)
+            hipLaunchKernelGGL((k_raycast<__half, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
+                               colors, depth, ub, tiles_x, n_tiles, fp);
+        else
+            hipLaunchKernelGGL((k_raycast<__half, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
+                               colors, depth, ub, tiles_x, n_tiles, fp);
+    } else {
+        if (color)
+            hipLaunchKernelGGL((k_raycast<float, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
+                               colors, depth, ub, tiles_x, n_tiles, fp);
+        else
+            hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
+                               colors, depth, ub, tiles_x, n_tiles, fp);
+    }
 }
 
 }  // namespace
@@ -1187,6 +1333,14 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->op_mask);
     (void)hipFree(g->ptmap_batch);
     (void)hipFree(g->ops_dev);
+    (void)hipFree(g->fp_bits);
+    (void)hipFree(g->fp_keys);
+    (void)hipFree(g->fp_counts);
+    (void)hipFree(g->fp_points);
+    for (int k = 0; k < 2 * g->prof_cap; ++k)
+        (void)hipEventDestroy(g->prof_ev[k]);
+    free(g->prof_ev);
+    free(g->prof_kind);
     if (g->ctrl_host)
         (void)hipHostFree(g->ctrl_host);
     if (g->ops_host)
@@ -1201,7 +1355,10 @@ tsdf_status tsdf_reset(tsdf_grid* g, void* stream)
     DeviceScope ds(g->p.device);
     hipStream_t s = (hipStream_t)stream;
     const size_t words = g->op_mask ? (size_t)g->p.n_buckets * kMaskWords : 0;
-    hipLaunchKernelGGL(k_reset, dim3(g->n_cu * 4), dim3(kThreads), 0, s, view(g), g->op_mask, words);
+    {
+        ProfScope ps(g, s, TSDF_K_RESET);
+        hipLaunchKernelGGL(k_reset, dim3(g->n_cu * 4), dim3(kThreads), 0, s, view(g), g->op_mask, words);
+    }
     HIPC(hipGetLastError());
     g->epoch = 0;
     return TSDF_OK;
@@ -1248,12 +1405,18 @@ tsdf_status tsdf_integrate(tsdf_grid* g, const uint16_t* depth, int32_t pitch, c
     const uint32_t epoch = ++g->epoch;
     const int par = (int)(epoch & 1u);
     const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH);
-    hipLaunchKernelGGL(k_collect, cgrid, dim3(kThreads), 0, s, view(g), op, epoch, par);
+    {
+        ProfScope ps(g, s, TSDF_K_COLLECT);
+        hipLaunchKernelGGL(k_collect, cgrid, dim3(kThreads), 0, s, view(g), op, epoch, par);
+    }
     const bool color = g->p.with_color != 0;
-    if (g->p.storage == TSDF_STORAGE_F16)
-        launch_integrate<__half>(g, op, par, s, color);
-    else
-        launch_integrate<float>(g, op, par, s, color);
+    {
+        ProfScope ps(g, s, TSDF_K_INTEGRATE);
+        if (g->p.storage == TSDF_STORAGE_F16)
+            launch_integrate<__half>(g, op, par, s, color);
+        else
+            launch_integrate<float>(g, op, par, s, color);
+    }
     HIPC(hipGetLastError());
     return TSDF_OK;
 }
@@ -1301,12 +1464,18 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
         const uint32_t epoch = ++g->epoch;
         const int par = (int)(epoch & 1u);
         const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH, n);
-        hipLaunchKernelGGL(k_collect_batch, cgrid, dim3(kThreads), 0, s, view(g), g->ops_dev, epoch, par,
-                           g->op_mask);
-        if (g->p.storage == TSDF_STORAGE_F16)
-            launch_integrate_batch<__half>(g, par, s, color);
-        else
-            launch_integrate_batch<float>(g, par, s, color);
+        {
+            ProfScope ps(g, s, TSDF_K_COLLECT_BATCH);
+            hipLaunchKernelGGL(k_collect_batch, cgrid, dim3(kThreads), 0, s, view(g), g->ops_dev, epoch, par,
+                               g->op_mask);
+        }
+        {
+            ProfScope ps(g, s, TSDF_K_INTEGRATE_BATCH);
+            if (g->p.storage == TSDF_STORAGE_F16)
+                launch_integrate_batch<__half>(g, par, s, color);
+            else
+                launch_integrate_batch<float>(g, par, s, color);
+        }
         HIPC(hipGetLastError());
     }
     return TSDF_OK;
@@ -1352,27 +1521,84 @@ tsdf_status tsdf_raycast(tsdf_grid* g, const float E[16], float* points, float* 
     hipStream_t s = (hipStream_t)stream;
     Pose P;
     make_pose(E, &P);
-    const int tiles_x = (g->p.width + 15) / 16, tiles_y = (g->p.height + 15) / 16;
-    const int n_tiles = tiles_x * tiles_y;
-    const int grid = ((n_tiles + 7) / 8) * 8;
-    const bool color = g->p.with_color != 0;
-    float* cols = color ? colors : nullptr;
-    if (g->p.storage == TSDF_STORAGE_F16) {
-        if (color)
-            hipLaunchKernelGGL((k_raycast<__half, true>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points, cols,
-                               depth, ub_flags, tiles_x, n_tiles);
-        else
-            hipLaunchKernelGGL((k_raycast<__half, false>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points,
-                               colors, depth, ub_flags, tiles_x, n_tiles);
-    } else {
-        if (color)
-            hipLaunchKernelGGL((k_raycast<float, true>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points, cols,
-                               depth, ub_flags, tiles_x, n_tiles);
-        else
-            hipLaunchKernelGGL((k_raycast<float, false>), dim3(grid), dim3(kThreads), 0, s, view(g), P, points,
-                               colors, depth, ub_flags, tiles_x, n_tiles);
+    Footprint fp;
+    memset(&fp, 0, sizeof(fp));
+    {
+        ProfScope ps(g, s, TSDF_K_RAYCAST);
+        launch_raycast<false>(g, P, points, g->p.with_color ? colors : nullptr, depth, ub_flags, fp, s);
     }
     HIPC(hipGetLastError());
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_raycast_footprint(tsdf_grid* g, const float E[16], uint64_t* distinct_voxels,
+                                   uint64_t* distinct_keys, void* stream)
+{
+    if (!g || !E || !distinct_voxels || !distinct_keys)
+        return fail(TSDF_BAD_ARG, "NULL argument");
+    if (!g->intr_set)
+        return fail(TSDF_NO_INTRINSICS, "tsdf_set_intrinsics was not called");
+    if (!pose_ok(E))
+        return fail(TSDF_BAD_POSE, "pose must be finite and rigid with last row 0 0 0 1");
+    DeviceScope ds(g->p.device);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t words = (size_t)g->p.n_bricks * 16;
+    if (!g->fp_bits) {
+        g->fp_key_mask = (1u << 23) - 1u;
+        HIPC(hipMalloc(&g->fp_bits, words * sizeof(uint32_t)));
+        HIPC(hipMalloc(&g->fp_keys, ((size_t)g->fp_key_mask + 1) * sizeof(unsigned long long)));
+        HIPC(hipMalloc(&g->fp_counts, 4 * sizeof(unsigned long long)));
+        HIPC(hipMalloc(&g->fp_points, (size_t)g->p.height * g->p.width * 3 * sizeof(float)));
+    }
+    HIPC(hipMemsetAsync(g->fp_bits, 0, words * sizeof(uint32_t), s));
+    HIPC(hipMemsetAsync(g->fp_keys, 0xFF, ((size_t)g->fp_key_mask + 1) * sizeof(unsigned long long), s));
+    HIPC(hipMemsetAsync(g->fp_counts, 0, 4 * sizeof(unsigned long long), s));
+    Pose P;
+    make_pose(E, &P);
+    Footprint fp;
+    fp.vox_bits = g->fp_bits;
+    fp.keys = g->fp_keys;
+    fp.key_mask = g->fp_key_mask;
+    fp.counts = g->fp_counts;
+    launch_raycast<true>(g, P, g->fp_points, nullptr, nullptr, nullptr, fp, s);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_popcount, dim3(g->n_cu * 4), dim3(kThreads), 0, s, g->fp_bits, words, g->fp_counts + 2);
+    HIPC(hipGetLastError());
+    unsigned long long c[4];
+    HIPC(hipMemcpyAsync(c, g->fp_counts, sizeof(c), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (c[1])
+        return fail(TSDF_MAP_FULL, "footprint key set overflowed: " + std::to_string(c[0]) + " keys inserted, " +
+                                       std::to_string(c[1]) + " refused");
+    *distinct_voxels = c[2];
+    *distinct_keys = c[0];
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_profile(tsdf_grid* g, int32_t enable)
+{
+    if (!g)
+        return fail(TSDF_BAD_ARG, "NULL grid");
+    g->prof_on = enable != 0;
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_profile_read(tsdf_grid* g, tsdf_kernel_times* out, int32_t reset)
+{
+    if (!g || !out)
+        return fail(TSDF_BAD_ARG, "NULL grid or output");
+    DeviceScope ds(g->p.device);
+    tsdf_status st = prof_flush(g);
+    if (st != TSDF_OK)
+        return st;
+    for (int k = 0; k < TSDF_N_KERNEL_KINDS; ++k) {
+        out->ms[k] = g->prof_ms[k];
+        out->launches[k] = g->prof_n[k];
+        if (reset) {
+            g->prof_ms[k] = 0.0;
+            g->prof_n[k] = 0;
+        }
+    }
     return TSDF_OK;
 }
 

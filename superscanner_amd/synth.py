@@ -154,10 +154,12 @@ def scene_room(seed=1, size=(5.0, 4.0, 2.6)):
     return prims
 
 
-def traj_random_walk(n=600, seed=2, step_m=0.005, step_deg=0.5, start=(0.0, 0.0, 1.4)):
+def traj_random_walk(n=600, seed=2, step_m=0.005, step_deg=0.5, start=(0.0, 0.0, 1.4),
+                     bounds=((-1.6, 1.6), (-1.2, 1.2), (1.0, 1.8))):
     """Random walk: each step moves 5 mm along a heading that turns by at most
-    step_deg, and rotates the camera by step_deg about a random axis biased
-    toward the world vertical; the first pose looks horizontally along +x."""
+    step_deg (reflected at `bounds`, so the camera stays inside the room), and
+    rotates the camera by step_deg about a random axis biased toward the
+    world vertical; the first pose looks along +x, slightly down."""
     rng = np.random.default_rng(seed)
     T = look_at(start, (start[0] + 1.0, start[1], start[2] - 0.3))
     heading = np.array([1.0, 0.3, 0.0])
@@ -167,6 +169,10 @@ def traj_random_walk(n=600, seed=2, step_m=0.005, step_deg=0.5, start=(0.0, 0.0,
         poses.append(T.copy())
         turn = axis_angle((0, 0, 1), math.radians(rng.uniform(-step_deg, step_deg)))
         heading = turn @ heading
+        nxt = T[:3, 3] + step_m * heading
+        for a in range(3):
+            if not bounds[a][0] <= nxt[a] <= bounds[a][1]:
+                heading[a] = -heading[a]
         T[:3, 3] += step_m * heading
         ax = rng.normal(size=3) * np.array([0.3, 0.3, 1.0])
         T[:3, :3] = axis_angle(ax, math.radians(step_deg)) @ T[:3, :3]

@@ -220,6 +220,23 @@ class GPUVoxelGrid:
         check(self._L.tsdf_stats(self._h, C.byref(s), _stream_ptr(stream)), "tsdf_stats")
         return s.as_dict()
 
+    def raycast_footprint(self, extr, stream=None) -> dict:
+        """Distinct voxels read and distinct keys probed by the raycast from
+        `extr` (diagnostic, synchronous)."""
+        keep, E = _pose_ptr(extr)
+        nv, nk = C.c_uint64(0), C.c_uint64(0)
+        check(self._L.tsdf_raycast_footprint(self._h, E, C.byref(nv), C.byref(nk), _stream_ptr(stream)),
+              "tsdf_raycast_footprint")
+        return {"u_vox": int(nv.value), "u_blk": int(nk.value)}
+
+    def profile(self, enable=True):
+        check(self._L.tsdf_profile(self._h, int(bool(enable))), "tsdf_profile")
+
+    def profile_read(self, reset=True) -> dict:
+        t = _lib.TsdfKernelTimes()
+        check(self._L.tsdf_profile_read(self._h, C.byref(t), int(bool(reset))), "tsdf_profile_read")
+        return t.as_dict()
+
     def size(self) -> int:
         """voxelMap2.size()"""
         return self.stats()["n_bricks"]

@@ -80,6 +80,10 @@ def test_plain_small_full_size(gpu):
     # the march itself matches: same number of voxel reads and hash lookups
     assert after["ray_samples"] - before["ray_samples"] == samples
     assert after["ray_probes"] - before["ray_probes"] == probes
+    # the footprint counters used for the raycast roofline equal the oracle's
+    fp = g.raycast_footprint(poses[7])
+    r = o.raycast(poses[7], stats=True)["counts"]
+    assert fp["u_vox"] == r["u_vox"] and fp["u_blk"] == r["u_blk"], (fp, r)
 
 
 def test_deintegrate_sequence(gpu):
