@@ -103,6 +103,14 @@ typedef struct {
     uint64_t launches[TSDF_N_KERNEL_KINDS]; /* launches timed per kind       */
 } tsdf_kernel_times;
 
+/* Launch shapes of the persistent kernels (no reference counterpart; picked
+ * per device and workload by superscanner_amd.autotune). */
+typedef struct {
+    int32_t raycast_xcd_map;      /* 1: each XCD takes one band of the image    */
+    int32_t raycast_superblocks;  /* 1: skip empty superblocks when certified   */
+    int32_t integrate_wg_per_cu;  /* brick-update workgroups per CU            */
+} tsdf_tuning;
+
 /* Fills the defaults of the reference: d_max 10 m, rmin 0.2 m, no weight
  * cap, fp32 storage, no colour (GPUVoxelGrid.h:55-62). */
 void tsdf_default_params(tsdf_params *p);
@@ -157,6 +165,11 @@ tsdf_status tsdf_reintegrate_batch(tsdf_grid *g, int32_t n, const uint16_t *cons
  * point or NULL (extension).  ub_flags: device [H][W] uint8 or NULL. */
 tsdf_status tsdf_raycast(tsdf_grid *g, const float E_cam2world[16], float *points,
                          float *colors, float *depth, uint8_t *ub_flags, void *stream);
+
+/* Defaults, current values and replacement of the launch shapes. */
+void tsdf_default_tuning(tsdf_tuning *t);
+tsdf_status tsdf_get_tuning(const tsdf_grid *g, tsdf_tuning *t);
+tsdf_status tsdf_set_tuning(tsdf_grid *g, const tsdf_tuning *t);
 
 /* Diagnostic raycast from E that counts the distinct voxels read and the
  * distinct block keys probed (hits and misses) by the march of

@@ -33,6 +33,7 @@ EXPORTS = (
     "tsdf_reintegrate_batch", "tsdf_raycast", "tsdf_sync", "tsdf_stats",
     "tsdf_export_bricks", "tsdf_pose_prepare", "tsdf_status_string",
     "tsdf_last_error", "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
+    "tsdf_default_tuning", "tsdf_get_tuning", "tsdf_set_tuning",
 )
 
 KERNEL_KINDS = ("reset", "collect", "integrate", "collect_batch", "integrate_batch", "raycast")
@@ -60,6 +61,14 @@ class TsdfStats(C.Structure):
         "n_bricks", "capacity", "overflow", "last_touched", "total_touched",
         "total_new", "total_updated", "total_valid_px", "ray_hits", "ray_ub",
         "ray_samples", "ray_probes")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class TsdfTuning(C.Structure):
+    _fields_ = [("raycast_xcd_map", C.c_int32), ("raycast_superblocks", C.c_int32),
+                ("integrate_wg_per_cu", C.c_int32)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -112,6 +121,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_pose_prepare.argtypes = [fp, fp, fp]
     L.tsdf_raycast_footprint.argtypes = [vp, fp, P(C.c_uint64), P(C.c_uint64), vp]
     L.tsdf_profile.argtypes = [vp, C.c_int32]
+    L.tsdf_default_tuning.argtypes = [P(TsdfTuning)]
+    L.tsdf_default_tuning.restype = None
+    L.tsdf_get_tuning.argtypes = [vp, P(TsdfTuning)]
+    L.tsdf_set_tuning.argtypes = [vp, P(TsdfTuning)]
     L.tsdf_profile_read.argtypes = [vp, P(TsdfKernelTimes), C.c_int32]
     L.tsdf_status_string.argtypes = [C.c_int]
     L.tsdf_status_string.restype = C.c_char_p
@@ -120,7 +133,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     for name in ("tsdf_create", "tsdf_reset", "tsdf_set_intrinsics", "tsdf_integrate",
                  "tsdf_integrate_batch", "tsdf_reintegrate_batch", "tsdf_raycast",
                  "tsdf_sync", "tsdf_stats", "tsdf_export_bricks", "tsdf_pose_prepare",
-                 "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read"):
+                 "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
+                 "tsdf_get_tuning", "tsdf_set_tuning"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

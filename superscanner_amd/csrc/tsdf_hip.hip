@@ -27,6 +27,7 @@
 
 #include "../../include/tsdf.h"
 #include "tsdf_math.hpp"
+#include "tsdf_march.hpp"
 
 #pragma clang fp contract(off)
 
@@ -38,34 +39,7 @@ namespace {
 // device-side state
 // ---------------------------------------------------------------------------
 
-struct Ctrl {
-    unsigned int alloc_top;         // brick ids handed out (may pass n_bricks)
-    unsigned int overflow;          // refused allocations / full table
-    unsigned int touched_count[2];  // touched-list length, by call parity
-    unsigned long long total_touched;
-    unsigned long long total_new;
-    unsigned long long total_updated;
-    unsigned long long total_valid;
-    unsigned long long ray_hits;
-    unsigned long long ray_ub;
-    unsigned long long ray_samples;
-    unsigned long long ray_probes;
-};
 
-struct GridView {
-    unsigned long long* slots;
-    uint32_t* slot_epoch;
-    int4* coords;
-    void* payload;
-    float* rgb;
-    uint32_t* touched;
-    Ctrl* ctrl;
-    uint32_t mask;      // n_buckets - 1
-    uint32_t n_bricks;
-    float vl, vbl, trunc, rmin, dmax, wmax;
-    int H, W;
-    Camera cam;
-};
 
 // One integrate (+1) or deintegrate (-1) operation.
 struct OpDesc {
@@ -154,9 +128,22 @@ __device__ int find_or_insert(const GridView& g, uint32_t key, int x, int y, int
                     g.coords[id] = make_int4(x, y, z, 1);
                     idv = id;
                     atomicAdd(&g.ctrl->total_new, 1ull);
-                } else {
-                    atomicAdd(&g.ctrl->overflow, 1u);
                 }
+                {
+                    // every inserted key counts for the raycast's empty-space
+                    // certificate, whether or not its brick was allocated
+                    const int sb = sb_index(x, y, z);
+                    if (sb >= 0)
+                        atomicOr(&g.sb_bits[sb >> 5], 1u << (sb & 31));
+                    atomicMin(&g.ctrl->bbox_min[0], x);
+                    atomicMin(&g.ctrl->bbox_min[1], y);
+                    atomicMin(&g.ctrl->bbox_min[2], z);
+                    atomicMax(&g.ctrl->bbox_max[0], x);
+                    atomicMax(&g.ctrl->bbox_max[1], y);
+                    atomicMax(&g.ctrl->bbox_max[2], z);
+                }
+                if (id >= g.n_bricks)
+                    atomicAdd(&g.ctrl->overflow, 1u);
                 atomicExch(&g.slots[b], keyhi | idv);
                 return (int)b;
             }
@@ -164,24 +151,6 @@ __device__ int find_or_insert(const GridView& g, uint32_t key, int x, int y, int
         }
         if ((uint32_t)(s >> 32) == key)
             return (int)b;
-        b = (b + 1) & g.mask;
-    }
-    return -1;
-}
-
-// Brick id of block (x, y, z) or -1 (voxelMap2.find + valid_at).
-__device__ __forceinline__ int find_brick(const GridView& g, int x, int y, int z)
-{
-    const uint32_t key = prime_hash(x, y, z);
-    uint32_t b = mix32(key) & g.mask;
-    for (uint32_t p = 0; p <= g.mask; ++p) {
-        const unsigned long long s = g.slots[b];
-        if (s == kSlotEmpty)
-            return -1;
-        if ((uint32_t)(s >> 32) == key) {
-            const uint32_t id = (uint32_t)s;
-            return id < g.n_bricks ? (int)id : -1;
-        }
         b = (b + 1) & g.mask;
     }
     return -1;
@@ -201,6 +170,8 @@ __device__ __forceinline__ bool claim(const GridView& g, int x, int y, int z, ui
     if (kBatch)
         atomicOr(&op_mask[(size_t)s * kMaskWords + (op >> 5)], 1u << (op & 31));
     *slot_out = (uint32_t)s;
+    if (__hip_atomic_load(&g.slot_epoch[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+        return false;  // another tile of this call claimed it already
     return atomicExch(&g.slot_epoch[s], epoch) != epoch;
 }
 
@@ -208,6 +179,13 @@ __device__ __forceinline__ bool lds_insert(unsigned long long* set, unsigned lon
 {
     uint32_t h = mix32((uint32_t)v ^ ((uint32_t)(v >> 32) * 0x9E3779B1u)) & (kLdsSet - 1);
     for (int p = 0; p < 32; ++p) {
+        const unsigned long long cur = set[h];  // most candidates are already present
+        if (cur == v)
+            return true;
+        if (cur != kSlotEmpty) {
+            h = (h + 1) & (kLdsSet - 1);
+            continue;
+        }
         const unsigned long long old = atomicCAS(&set[h], kSlotEmpty, v);
         if (old == kSlotEmpty || old == v)
             return true;
@@ -231,9 +209,15 @@ __global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
     if (op_mask)
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < op_mask_words; i += stride)
             op_mask[i] = 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < kSbWords; i += stride)
+        g.sb_bits[i] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Ctrl z;
         memset(&z, 0, sizeof(z));
+        for (int a = 0; a < 3; ++a) {
+            z.bbox_min[a] = 0x7FFFFFFF;
+            z.bbox_max[a] = (int)0x80000000u;
+        }
         *g.ctrl = z;
     }
 }
@@ -274,9 +258,9 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
             ++nvalid;
             const f3 pt = xform(p, op.pose.E);
             o = make_float4(pt.x, pt.y, pt.z, 1.0f);
-            const int x0 = vb_index(pt.x - g.trunc, g.vbl), x1 = vb_index(pt.x + g.trunc, g.vbl);
-            const int y0 = vb_index(pt.y - g.trunc, g.vbl), y1 = vb_index(pt.y + g.trunc, g.vbl);
-            const int z0 = vb_index(pt.z - g.trunc, g.vbl), z1 = vb_index(pt.z + g.trunc, g.vbl);
+            const int x0 = div_index(pt.x - g.trunc, g.vbl, g.rvbl), x1 = div_index(pt.x + g.trunc, g.vbl, g.rvbl);
+            const int y0 = div_index(pt.y - g.trunc, g.vbl, g.rvbl), y1 = div_index(pt.y + g.trunc, g.vbl, g.rvbl);
+            const int z0 = div_index(pt.z - g.trunc, g.vbl, g.rvbl), z1 = div_index(pt.z + g.trunc, g.vbl, g.rvbl);
             const bool sane = (long long)x1 - x0 < kMaxAxisBlocks && (long long)y1 - y0 < kMaxAxisBlocks &&
                               (long long)z1 - z0 < kMaxAxisBlocks;
             if (sane) {
@@ -304,13 +288,35 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
     if (threadIdx.x == 0 && s_valid)
         atomicAdd(&g.ctrl->total_valid, (unsigned long long)s_valid);
 
+    // compact the tile's distinct blocks, then claim them all in one round
+    // (one dependent chain of global accesses per tile rather than one per
+    // stripe of the set)
+    __shared__ unsigned long long uniq[kLdsSet];
+    __shared__ unsigned int n_uniq;
+    if (threadIdx.x == 0)
+        n_uniq = 0;
+    __syncthreads();
     for (int e = threadIdx.x; e < kLdsSet; e += kThreads) {
         const unsigned long long v = lset[e];
+        const bool has = v != kSlotEmpty;
+        const unsigned long long m = __ballot(has);
+        unsigned int base = 0;
+        if (__lane_id() == 0 && m)
+            base = atomicAdd(&n_uniq, (unsigned int)__popcll(m));
+        base = __shfl(base, 0);
+        if (has)
+            uniq[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = v;
+    }
+    __syncthreads();
+    const unsigned int nu = n_uniq;
+    for (unsigned int e = threadIdx.x; e < ((nu + 63u) & ~63u); e += kThreads) {
         bool need = false;
         uint32_t slot = 0;
-        if (v != kSlotEmpty)
-            need = claim<kBatch>(g, unpack21(v >> 42), unpack21(v >> 21), unpack21(v), epoch, op_mask,
-                                 op_index, &slot);
+        if (e < nu) {
+            const unsigned long long v = uniq[e];
+            need = claim<kBatch>(g, unpack21(v >> 42), unpack21(v >> 21), unpack21(v), epoch, op_mask, op_index,
+                                 &slot);
+        }
         wave_append(need, slot, &g.ctrl->touched_count[par], g.touched);
     }
 }
@@ -330,6 +336,40 @@ __global__ __launch_bounds__(kThreads) void k_collect_batch(GridView g, const Op
 // per-voxel update (updateSingleVoxel[Color] / deUpdateSingleVoxel[Color],
 // GPUVoxelGrid.cpp:345-469) on register-resident values
 // ---------------------------------------------------------------------------
+
+// The running-average update of one voxel by one measurement sdf (and pixel
+// colour px), GPUVoxelGrid.cpp:360-361, 384-391, 412-420, 450-464.
+template <bool kColor>
+__device__ __forceinline__ void apply_update(const GridView& g, const OpDesc& op, float sdf, const float* px,
+                                             float& s, float& w, float* c)
+{
+    if (op.sign > 0) {
+        s = (s * w + sdf) / (w + 1.0f);
+        if (kColor && op.rgb) {
+            c[0] = (c[0] * w + px[0]) / (w + 1.0f);
+            c[1] = (c[1] * w + px[1]) / (w + 1.0f);
+            c[2] = (c[2] * w + px[2]) / (w + 1.0f);
+        }
+        // Weight cap: project semantics, inactive at max_weight == 0 (the
+        // reference never caps, GPUVoxelGrid.h:14,62).
+        w = (g.wmax > 0.0f && w >= g.wmax) ? g.wmax : w + 1.0f;
+    } else {
+        if (w != 1.0f) {
+            s = (s * w - sdf) / (w - 1.0f);
+            if (kColor && op.rgb) {
+                c[0] = (c[0] * w - px[0]) / (w - 1.0f);
+                c[1] = (c[1] * w - px[1]) / (w - 1.0f);
+                c[2] = (c[2] * w - px[2]) / (w - 1.0f);
+            }
+            w = w - 1.0f;
+        } else {
+            w = 0.0f;
+            s = 0.0f;
+            if (kColor && op.rgb)
+                c[0] = c[1] = c[2] = 0.0f;
+        }
+    }
+}
 
 template <bool kColor>
 __device__ __forceinline__ bool update_voxel(const GridView& g, const OpDesc& op, float vmx, float vmy,
@@ -360,32 +400,7 @@ __device__ __forceinline__ bool update_voxel(const GridView& g, const OpDesc& op
         px[1] = (float)q[1];
         px[2] = (float)q[2];
     }
-    if (op.sign > 0) {
-        s = (s * w + sdf) / (w + 1.0f);
-        if (kColor && op.rgb) {
-            c[0] = (c[0] * w + px[0]) / (w + 1.0f);
-            c[1] = (c[1] * w + px[1]) / (w + 1.0f);
-            c[2] = (c[2] * w + px[2]) / (w + 1.0f);
-        }
-        // Weight cap: project semantics, inactive at max_weight == 0 (the
-        // reference never caps, GPUVoxelGrid.h:14,62).
-        w = (g.wmax > 0.0f && w >= g.wmax) ? g.wmax : w + 1.0f;
-    } else {
-        if (w != 1.0f) {
-            s = (s * w - sdf) / (w - 1.0f);
-            if (kColor && op.rgb) {
-                c[0] = (c[0] * w - px[0]) / (w - 1.0f);
-                c[1] = (c[1] * w - px[1]) / (w - 1.0f);
-                c[2] = (c[2] * w - px[2]) / (w - 1.0f);
-            }
-            w = w - 1.0f;
-        } else {
-            w = 0.0f;
-            s = 0.0f;
-            if (kColor && op.rgb)
-                c[0] = c[1] = c[2] = 0.0f;
-        }
-    }
+    apply_update<kColor>(g, op, sdf, px, s, w, c);
     return true;
 }
 
@@ -415,62 +430,146 @@ __device__ __forceinline__ WaveRange wave_range(unsigned int count)
 // k_integrate: integrationCore[Color]Tag / deIntegrationCore[Color]Tag,
 // GPUVoxelGrid.cpp:516-550.  One wavefront per touched brick, lane = (x, y),
 // eight voxels along z per lane.
+//
+// A workgroup takes the touched list in chunks of kMetaChunk entries and
+// first resolves slot -> brick id -> block index for the whole chunk at once
+// (three dependent loads per chunk rather than per brick), then its four
+// wavefronts update the chunk's bricks.  Within a brick, the eight payload
+// loads and the eight depth-point gathers of a lane are issued together.
 // ---------------------------------------------------------------------------
+constexpr int kMetaChunk = 16;
+
+struct BrickMeta {
+    int id;  // brick id, or -1 when the slot's allocation failed
+    int x, y, z;
+    bool fresh;
+};
+
+// Resolves touched[t0 .. t0 + kMetaChunk) into LDS (threads 0..63).
+__device__ __forceinline__ void load_meta(const GridView& g, unsigned int t0, unsigned int t1, int4* meta)
+{
+    if (threadIdx.x < kMetaChunk) {
+        const unsigned int t = t0 + threadIdx.x;
+        int4 m = make_int4(-1, 0, 0, 0);
+        if (t < t1) {
+            const uint32_t id = (uint32_t)g.slots[g.touched[t]];
+            if (id < g.n_bricks) {
+                const int4 c = g.coords[id];
+                m = make_int4((int)(id | ((uint32_t)(c.w & 1) << 31)), c.x, c.y, c.z);
+                if (c.w & 1)
+                    g.coords[id].w = c.w & ~1;  // the brick is written below
+            }
+        }
+        meta[threadIdx.x] = m;
+    }
+}
+
+template <typename T, bool kColor>
+__device__ __forceinline__ unsigned int integrate_brick(const GridView& g, const OpDesc& op, const Offsets& off,
+                                                        int4 m, int lane)
+{
+    const bool fresh = m.x < 0;  // bit 31: allocated by this call, never written
+    const int id = m.x & 0x7FFFFFFF;
+    const int x = lane & 7, y = lane >> 3;
+    T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
+    float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
+    float s[kVPE], w[kVPE], col[kVPE][3];
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z) {
+        s[z] = w[z] = 0.f;
+        col[z][0] = col[z][1] = col[z][2] = 0.f;
+        if (!fresh) {
+            ld_pair(sd, lane + 64 * z, s[z], w[z]);
+            if (kColor) {
+                col[z][0] = cr[lane + 64 * z];
+                col[z][1] = cr[lane + 64 * z + kVPB];
+                col[z][2] = cr[lane + 64 * z + 2 * kVPB];
+            }
+        }
+    }
+    const float pzx = g.vbl * (float)m.y, pzy = g.vbl * (float)m.z, pzz = g.vbl * (float)m.w;
+    const float vmx = (float)((double)pzx + off.v[x]);
+    const float vmy = (float)((double)pzy + off.v[y]);
+    const float* Ei = op.pose.Ei;
+    const float a0 = vmx * Ei[0] + vmy * Ei[1];
+    const float a1 = vmx * Ei[4] + vmy * Ei[5];
+    const float a2 = vmx * Ei[8] + vmy * Ei[9];
+    // projection of every voxel (transformPoint(vm, E_inv) + getCoordinate,
+    // GPUVoxelGrid.cpp:347-350), then all gathers of inputPcd at once
+    float vmz[kVPE];
+    int pix[kVPE];
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z) {
+        vmz[z] = (float)((double)pzz + off.v[z]);
+        const float cx = a0 + vmz[z] * Ei[2] + Ei[3];
+        const float cy = a1 + vmz[z] * Ei[6] + Ei[7];
+        const float cz = a2 + vmz[z] * Ei[10] + Ei[11];
+        const float row = cy * g.cam.fy / cz + g.cam.ci;
+        const float colf = cx * g.cam.fx / cz + g.cam.cj;
+        const int cori = f2i(row), corj = f2i(colf);
+        pix[z] = (cori < g.H && corj < g.W && cori >= 0 && corj >= 0) ? cori * g.W + corj : -1;
+    }
+    float4 pt[kVPE];
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z)
+        pt[z] = op.ptmap[pix[z] >= 0 ? pix[z] : 0];
+    unsigned int nupd = 0;
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z) {
+        const int v = lane + 64 * z;
+        bool upd = false;
+        if (pix[z] >= 0 && pt[z].w != 0.0f) {  // inputPcd(cori, corj).z > rmin
+            const float sdf = (pt[z].x - vmx) * op.pose.Ez[0] + (pt[z].y - vmy) * op.pose.Ez[1] +
+                              (pt[z].z - vmz[z]) * op.pose.Ez[2];
+            if (fabsf(sdf) < g.trunc) {
+                upd = true;
+                float px[3] = {0.f, 0.f, 0.f};
+                if (kColor && op.rgb) {
+                    const uint8_t* q = op.rgb + (size_t)pix[z] * 3;
+                    px[0] = (float)q[0];
+                    px[1] = (float)q[1];
+                    px[2] = (float)q[2];
+                }
+                apply_update<kColor>(g, op, sdf, px, s[z], w[z], col[z]);
+            }
+        }
+        if (upd || fresh) {
+            st_pair(sd, v, s[z], w[z]);
+            if (kColor) {
+                cr[v] = col[z][0];
+                cr[v + kVPB] = col[z][1];
+                cr[v + 2 * kVPB] = col[z][2];
+            }
+        }
+        nupd += upd ? 1u : 0u;
+    }
+    return nupd;
+}
+
 template <typename T, bool kColor>
 __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, int par, Offsets off)
 {
+    __shared__ int4 meta[kMetaChunk];
     const unsigned int count = g.ctrl->touched_count[par];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         g.ctrl->touched_count[par ^ 1] = 0u;  // ready for the next call
         g.ctrl->total_touched += count;
     }
-    const int lane = threadIdx.x & 63;
-    const int x = lane & 7, y = lane >> 3;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // each XCD walks one contiguous eighth of the touched list (spatial
+    // neighbours share its L2); its workgroups stride over that eighth
+    const unsigned int xcd = blockIdx.x & 7u, nwg = gridDim.x >> 3, wg = blockIdx.x >> 3;
+    const unsigned int seg = (count + 7u) >> 3;
+    const unsigned int sb = min(count, xcd * seg), se = min(count, sb + seg);
     unsigned int nupd = 0;
-    const WaveRange wr = wave_range(count);
-    for (unsigned int t = wr.begin; t < wr.end; t += wr.stride) {
-        const uint32_t id = (uint32_t)g.slots[g.touched[t]];
-        if (id >= g.n_bricks)
-            continue;
-        const int4 c = g.coords[id];
-        const bool fresh = (c.w & 1) != 0;
-        if (fresh && lane == 0)
-            g.coords[id].w = c.w & ~1;
-        const float pzx = g.vbl * (float)c.x, pzy = g.vbl * (float)c.y, pzz = g.vbl * (float)c.z;
-        const float vmx = (float)((double)pzx + off.v[x]);
-        const float vmy = (float)((double)pzy + off.v[y]);
-        const float* Ei = op.pose.Ei;
-        const float a0 = vmx * Ei[0] + vmy * Ei[1];
-        const float a1 = vmx * Ei[4] + vmy * Ei[5];
-        const float a2 = vmx * Ei[8] + vmy * Ei[9];
-        T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
-        T* wd = sd + kVPB;
-        float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
-#pragma unroll 2
-        for (int z = 0; z < kVPE; ++z) {
-            const int v = lane + 64 * z;
-            const float vmz = (float)((double)pzz + off.v[z]);
-            float s = 0.f, w = 0.f, col[3] = {0.f, 0.f, 0.f};
-            if (!fresh) {
-                s = ld(sd + v);
-                w = ld(wd + v);
-                if (kColor) {
-                    col[0] = cr[v];
-                    col[1] = cr[v + kVPB];
-                    col[2] = cr[v + 2 * kVPB];
-                }
-            }
-            const bool upd = update_voxel<kColor>(g, op, vmx, vmy, vmz, a0, a1, a2, s, w, col);
-            if (upd || fresh) {
-                st(sd + v, s);
-                st(wd + v, w);
-                if (kColor) {
-                    cr[v] = col[0];
-                    cr[v + kVPB] = col[1];
-                    cr[v + 2 * kVPB] = col[2];
-                }
-            }
-            nupd += upd ? 1u : 0u;
+    for (unsigned int t0 = sb + wg * kMetaChunk; t0 < se; t0 += nwg * kMetaChunk) {
+        __syncthreads();
+        load_meta(g, t0, se, meta);
+        __syncthreads();
+        for (int e = wave; e < kMetaChunk; e += kThreads / 64) {
+            const int4 m = meta[e];
+            if (m.x != -1)
+                nupd += integrate_brick<T, kColor>(g, op, off, m, lane);
         }
     }
     nupd = wave_sum(nupd);
@@ -515,7 +614,6 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
         const float vmx = (float)((double)pzx + off.v[x]);
         const float vmy = (float)((double)pzy + off.v[y]);
         T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
-        T* wd = sd + kVPB;
         float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
         float s[kVPE], w[kVPE], col[kVPE][3];
         float vmz[kVPE];
@@ -525,8 +623,10 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
             const int v = lane + 64 * z;
             vmz[z] = (float)((double)pzz + off.v[z]);
             dirty[z] = fresh;
-            s[z] = fresh ? 0.f : ld(sd + v);
-            w[z] = fresh ? 0.f : ld(wd + v);
+            s[z] = 0.f;
+            w[z] = 0.f;
+            if (!fresh)
+                ld_pair(sd, v, s[z], w[z]);
             col[z][0] = col[z][1] = col[z][2] = 0.f;
             if (kColor && !fresh) {
                 col[z][0] = cr[v];
@@ -557,8 +657,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
             if (!dirty[z])
                 continue;
             const int v = lane + 64 * z;
-            st(sd + v, s[z]);
-            st(wd + v, w[z]);
+            st_pair(sd, v, s[z], w[z]);
             if (kColor) {
                 cr[v] = col[z][0];
                 cr[v + kVPB] = col[z][1];
@@ -576,302 +675,73 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
 // 8 x 8 pixels per wavefront, 16 x 16 per workgroup.
 // ---------------------------------------------------------------------------
 
-template <typename T>
-struct Vox {
-    float s, w;
-};
-
-template <typename T>
-__device__ __forceinline__ Vox<T> read_voxel(const GridView& g, int id, int vx, int vy, int vz)
-{
-    const T* sd = (const T*)g.payload + (size_t)id * (2 * kVPB);
-    const int v = vx + kVPE * vy + kVPE * kVPE * vz;
-    Vox<T> r;
-    r.s = ld(sd + v);
-    r.w = ld(sd + kVPB + v);
-    return r;
-}
-
-__device__ __forceinline__ bool vin_ok(int x, int y, int z)
-{
-    return (unsigned)x < (unsigned)kVPE && (unsigned)y < (unsigned)kVPE && (unsigned)z < (unsigned)kVPE;
-}
-
-// Distinct voxels / keys a raycast reads (the compulsory-bytes terms of its
-// roofline); only the k_raycast<.., kStats = true> instantiation touches it.
-struct Footprint {
-    uint32_t* vox_bits;         // [n_bricks][16] one bit per voxel
-    unsigned long long* keys;   // open-addressing set of probed keys
-    uint32_t key_mask;
-    unsigned long long* counts; // [0] distinct keys, [1] set overflow, [2] distinct voxels
-};
-
-__device__ void fp_key(const Footprint& fp, uint32_t key)
-{
-    uint32_t b = mix32(key) & fp.key_mask;
-    for (int p = 0; p < 256; ++p) {
-        const unsigned long long s = fp.keys[b];
-        if (s == (unsigned long long)key)
-            return;
-        if (s == kSlotEmpty) {
-            const unsigned long long old = atomicCAS(&fp.keys[b], kSlotEmpty, (unsigned long long)key);
-            if (old == kSlotEmpty) {
-                atomicAdd(&fp.counts[0], 1ull);
-                return;
-            }
-            if (old == (unsigned long long)key)
-                return;
-        }
-        b = (b + 1) & fp.key_mask;
-    }
-    atomicAdd(&fp.counts[1], 1ull);
-}
-
-template <bool kStats>
-__device__ __forceinline__ int probe(const GridView& g, const Footprint& fp, int x, int y, int z,
-                                     unsigned int& probes)
-{
-    ++probes;
-    if (kStats)
-        fp_key(fp, prime_hash(x, y, z));
-    return find_brick(g, x, y, z);
-}
-
-template <bool kStats>
-__device__ __forceinline__ void note_voxel(const Footprint& fp, int id, int vx, int vy, int vz,
-                                           unsigned int& samples)
-{
-    ++samples;
-    if (kStats) {
-        const int v = vx + kVPE * vy + kVPE * kVPE * vz;
-        atomicOr(&fp.vox_bits[(size_t)id * 16 + (v >> 5)], 1u << (v & 31));
-    }
-}
-
-struct RayOut {
-    f3 p, c;
-    int code;  // 1 hit, 0 miss, -flag UB
-    unsigned int samples, probes;
-};
-
+// ---------------------------------------------------------------------------
+// k_raycast: raycastingTag, GPUVoxelGrid.cpp:561-715 (march in tsdf_march.hpp).
+// One pixel per lane, 8 x 8 pixels per wavefront, 16 x 16 per workgroup.
+// ---------------------------------------------------------------------------
 template <typename T, bool kColor, bool kStats>
-__device__ RayOut march(const GridView& g, const Pose& pose, int i, int j, const Footprint& fp)
+__global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, RayParams rp, float* points,
+                                                      float* colors, float* depth, uint8_t* ub, Footprint fp)
 {
-    RayOut out;
-    out.p = {0.f, 0.f, 0.f};
-    out.c = {0.f, 0.f, 0.f};
-    out.code = 0;
-    out.samples = 0;
-    out.probes = 0;
-    const float vl = g.vl, vbl = g.vbl, dt = g.trunc, dmax = g.dmax;
-    // createNormPcdTag, GPUVoxelGrid.cpp:717-722
-    f3 n = get_point(g.cam, i, j, 1.0f);
-    const float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
-    n.x /= len;
-    n.y /= len;
-    n.z /= len;
-    f3 rv = xform(n, pose.E);
-    const f3 dir = {rv.x - pose.E[3], rv.y - pose.E[7], rv.z - pose.E[11]};
-    float l = 0.1f;
-    const float a = 1.0f - l;
-    rv.x -= a * dir.x;
-    rv.y -= a * dir.y;
-    rv.z -= a * dir.z;
-
-    bool done = false, Fn = false, Lp = false;
-    int ox = 0, oy = 0, oz = 0;
-    while (!done && !Fn && l < dmax) {
-        int bx = vb_index(rv.x, vbl), by = vb_index(rv.y, vbl), bz = vb_index(rv.z, vbl);
-        int id = probe<kStats>(g, fp, bx, by, bz, out.probes);
-        int count = 0;
-        while (id < 0 && l < dmax) {
-            rv.x += vbl * dir.x;
-            rv.y += vbl * dir.y;
-            rv.z += vbl * dir.z;
-            l += vbl;
-            bx = vb_index(rv.x, vbl);
-            by = vb_index(rv.y, vbl);
-            bz = vb_index(rv.z, vbl);
-            id = probe<kStats>(g, fp, bx, by, bz, out.probes);
-            ++count;
-        }
-        if (l >= dmax)
-            break;
-        if (count > 0) {
-            const float back = vbl - dt;
-            rv.x -= back * dir.x;
-            rv.y -= back * dir.y;
-            rv.z -= back * dir.z;
-            l -= back;
-            bx = vb_index(rv.x, vbl);
-            by = vb_index(rv.y, vbl);
-            bz = vb_index(rv.z, vbl);
-            id = probe<kStats>(g, fp, bx, by, bz, out.probes);
-            while (id < 0 && l < dmax) {
-                rv.x += dt * dir.x;
-                rv.y += dt * dir.y;
-                rv.z += dt * dir.z;
-                l += dt;
-                bx = vb_index(rv.x, vbl);
-                by = vb_index(rv.y, vbl);
-                bz = vb_index(rv.z, vbl);
-                id = probe<kStats>(g, fp, bx, by, bz, out.probes);
-            }
-        }
-        if (l >= dmax)
-            break;
-        const int cur = id;
-        int vx = vin_index(rv.x, bx, vbl, vl), vy = vin_index(rv.y, by, vbl, vl), vz = vin_index(rv.z, bz, vbl, vl);
-        ox = bx;
-        oy = by;
-        oz = bz;
-        while (!done && l < dmax && ox == bx && oy == by && oz == bz && !Fn) {
-            if (!vin_ok(vx, vy, vz)) {
-                out.code = -TSDF_RAY_UB_VOXEL_INDEX;
-                return out;
-            }
-            const Vox<T> cv = read_voxel<T>(g, cur, vx, vy, vz);
-            note_voxel<kStats>(fp, cur, vx, vy, vz, out.samples);
-            if (cv.w > 0) {
-                if (cv.s > 0) {
-                    Lp = true;
-                    rv.x += vl * dir.x;
-                    rv.y += vl * dir.y;
-                    rv.z += vl * dir.z;
-                    l += vl;
-                    ox = bx;
-                    oy = by;
-                    oz = bz;
-                    bx = vb_index(rv.x, vbl);
-                    by = vb_index(rv.y, vbl);
-                    bz = vb_index(rv.z, vbl);
-                    vx = vin_index(rv.x, bx, vbl, vl);
-                    vy = vin_index(rv.y, by, vbl, vl);
-                    vz = vin_index(rv.z, bz, vbl, vl);
-                } else if (Lp) {
-                    done = true;
-                    const f3 vm_in = {voxel_mid(vx, bx, vbl, vl), voxel_mid(vy, by, vbl, vl),
-                                      voxel_mid(vz, bz, vbl, vl)};
-                    const int ivx = vx, ivy = vy, ivz = vz;
-                    rv.x -= vl * dir.x;
-                    rv.y -= vl * dir.y;
-                    rv.z -= vl * dir.z;
-                    l -= vl;
-                    bx = vb_index(rv.x, vbl);
-                    by = vb_index(rv.y, vbl);
-                    bz = vb_index(rv.z, vbl);
-                    vx = vin_index(rv.x, bx, vbl, vl);
-                    vy = vin_index(rv.y, by, vbl, vl);
-                    vz = vin_index(rv.z, bz, vbl, vl);
-                    const f3 vm_out = {voxel_mid(vx, bx, vbl, vl), voxel_mid(vy, by, vbl, vl),
-                                       voxel_mid(vz, bz, vbl, vl)};
-                    int ob = cur;
-                    if (bx != ox || by != oy || bz != oz) {
-                        ob = probe<kStats>(g, fp, bx, by, bz, out.probes);
-                        if (ob < 0) {
-                            out.code = -TSDF_RAY_UB_MISSING_OUT;
-                            return out;
-                        }
-                    }
-                    if (!vin_ok(vx, vy, vz)) {
-                        out.code = -TSDF_RAY_UB_VOXEL_INDEX;
-                        return out;
-                    }
-                    const Vox<T> vo = read_voxel<T>(g, ob, vx, vy, vz);
-                    note_voxel<kStats>(fp, ob, vx, vy, vz, out.samples);
-                    const float xx = (vo.w * vo.s / vl + cv.w * (1.0f - cv.s / vl)) / (cv.w + vo.w);
-                    f3 p;
-                    p.x = vm_out.x + xx * (vm_in.x - vm_out.x);
-                    p.y = vm_out.y + xx * (vm_in.y - vm_out.y);
-                    p.z = vm_out.z + xx * (vm_in.z - vm_out.z);
-                    const float sc = dot3(p, dir) - dot3(rv, dir);
-                    out.p.x = rv.x + sc * dir.x;
-                    out.p.y = rv.y + sc * dir.y;
-                    out.p.z = rv.z + sc * dir.z;
-                    if (kColor) {
-                        const float* co = g.rgb + (size_t)ob * (3 * kVPB);
-                        const float* cc = g.rgb + (size_t)cur * (3 * kVPB);
-                        const int vo_i = vx + kVPE * vy + kVPE * kVPE * vz;
-                        const int vi_i = ivx + kVPE * ivy + kVPE * kVPE * ivz;
-                        out.c.x = (float)((0.5 * (double)co[vo_i] + 0.5 * (double)cc[vi_i]) / 255.0);
-                        out.c.y = (float)((0.5 * (double)co[vo_i + kVPB] + 0.5 * (double)cc[vi_i + kVPB]) / 255.0);
-                        out.c.z =
-                            (float)((0.5 * (double)co[vo_i + 2 * kVPB] + 0.5 * (double)cc[vi_i + 2 * kVPB]) / 255.0);
-                    }
-                } else {
-                    Fn = true;
-                }
-            } else {
-                Lp = false;
-                rv.x += vl * dir.x;
-                rv.y += vl * dir.y;
-                rv.z += vl * dir.z;
-                l += vl;
-                ox = bx;
-                oy = by;
-                oz = bz;
-                bx = vb_index(rv.x, vbl);
-                by = vb_index(rv.y, vbl);
-                bz = vb_index(rv.z, vbl);
-                vx = vin_index(rv.x, bx, vbl, vl);
-                vy = vin_index(rv.y, by, vbl, vl);
-                vz = vin_index(rv.z, bz, vbl, vl);
-            }
-        }
-    }
-    out.code = done ? 1 : 0;
-    return out;
-}
-
-template <typename T, bool kColor, bool kStats>
-__global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, float* points, float* colors,
-                                                      float* depth, uint8_t* ub, int tiles_x, int n_tiles,
-                                                      Footprint fp)
-{
-    __shared__ unsigned int s_cnt[4];
-    if (threadIdx.x < 4)
-        s_cnt[threadIdx.x] = 0u;
-    __syncthreads();
-    // XCD-aware tile order: workgroups b, b+8, b+16, ... share an XCD and get
-    // consecutive tiles, so one XCD's L2 serves one band of the image.
+    const int tiles_x = (rp.W + 15) / 16;
+    const int n_tiles = tiles_x * ((rp.H + 15) / 16);
+    // Workgroups are dealt to the 8 XCDs round-robin; with xcd_map the
+    // workgroups of one XCD take one contiguous band of tiles, so the bricks
+    // one XCD's L2 holds belong to neighbouring rays.
     const int per = (n_tiles + 7) >> 3;
-    const int tile = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    const int tile = rp.xcd_map ? (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = (tile / tiles_x) * 16 + (wv >> 1) * 8 + (lane >> 3);
     const int j = (tile % tiles_x) * 16 + (wv & 1) * 8 + (lane & 7);
-    if (tile < n_tiles && i < g.H && j < g.W) {
-        const RayOut r = march<T, kColor, kStats>(g, pose, i, j, fp);
-        const size_t o = (size_t)i * g.W + j;
-        f3 p = r.p, c = r.c;
-        if (r.code < 0) {
-            p = {0.f, 0.f, 0.f};
-            c = {0.f, 0.f, 0.f};
+    // empty-space certificate for this raycast (identical in every lane):
+    // probed blocks lie within `reach` blocks of the camera's block
+    RayParams rq = rp;
+    if (kStats) {
+        rq.use_sb = 0;  // the footprint must see every probed key
+    } else if (rq.use_sb) {
+        const int reach = (int)(rp.dmax / rp.vbl) + 3;
+        const float* Ep = pose.E;
+        int c[3] = {vb_index(Ep[3], rp.vbl), vb_index(Ep[7], rp.vbl), vb_index(Ep[11], rp.vbl)};
+        int plo[3], phi[3], bmin[3], bmax[3];
+        for (int a = 0; a < 3; ++a) {
+            plo[a] = c[a] - reach;
+            phi[a] = c[a] + reach;
+            bmin[a] = g.ctrl->bbox_min[a];
+            bmax[a] = g.ctrl->bbox_max[a];
         }
-        points[o * 3 + 0] = p.x;
-        points[o * 3 + 1] = p.y;
-        points[o * 3 + 2] = p.z;
+        rq.use_sb = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
+    }
+    unsigned int hit = 0, bad = 0, samples = 0, probes = 0;
+    if (tile < n_tiles && i < rp.H && j < rp.W) {
+        const RayResult r = march<T, kColor, kStats>(g, pose, rq, i, j, fp);
+        const size_t o = (size_t)i * rp.W + j;
+        points[o * 3 + 0] = r.p.x;
+        points[o * 3 + 1] = r.p.y;
+        points[o * 3 + 2] = r.p.z;
         if (colors) {
-            colors[o * 3 + 0] = c.x;
-            colors[o * 3 + 1] = c.y;
-            colors[o * 3 + 2] = c.z;
+            colors[o * 3 + 0] = r.c.x;
+            colors[o * 3 + 1] = r.c.y;
+            colors[o * 3 + 2] = r.c.z;
         }
         if (depth)
-            depth[o] = (r.code > 0) ? (p.x * pose.Ei[8] + p.y * pose.Ei[9] + p.z * pose.Ei[10] + pose.Ei[11]) : 0.0f;
+            depth[o] = r.code == 1 ? (r.p.x * pose.Ei[8] + r.p.y * pose.Ei[9] + r.p.z * pose.Ei[10] + pose.Ei[11])
+                                   : 0.0f;
         if (ub)
             ub[o] = (uint8_t)(r.code < 0 ? -r.code : 0);
-        if (r.code > 0)
-            atomicAdd(&s_cnt[0], 1u);
-        if (r.code < 0)
-            atomicAdd(&s_cnt[1], 1u);
-        atomicAdd(&s_cnt[2], r.samples);
-        atomicAdd(&s_cnt[3], r.probes);
+        hit = r.code == 1;
+        bad = r.code < 0;
+        samples = r.samples;
+        probes = r.probes;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_cnt[0]) atomicAdd(&g.ctrl->ray_hits, (unsigned long long)s_cnt[0]);
-        if (s_cnt[1]) atomicAdd(&g.ctrl->ray_ub, (unsigned long long)s_cnt[1]);
-        if (s_cnt[2]) atomicAdd(&g.ctrl->ray_samples, (unsigned long long)s_cnt[2]);
-        if (s_cnt[3]) atomicAdd(&g.ctrl->ray_probes, (unsigned long long)s_cnt[3]);
+    hit = wave_sum(hit);
+    bad = wave_sum(bad);
+    samples = wave_sum(samples);
+    probes = wave_sum(probes);
+    if (lane == 0) {
+        if (hit) atomicAdd(&g.ctrl->ray_hits, (unsigned long long)hit);
+        if (bad) atomicAdd(&g.ctrl->ray_ub, (unsigned long long)bad);
+        if (samples) atomicAdd(&g.ctrl->ray_samples, (unsigned long long)samples);
+        if (probes) atomicAdd(&g.ctrl->ray_probes, (unsigned long long)probes);
     }
 }
 
@@ -902,8 +772,11 @@ __global__ __launch_bounds__(kThreads) void k_export(GridView g, unsigned int n,
         }
         const bool fresh = (g.coords[b].w & 1) != 0;  // allocated, never written
         for (int v = threadIdx.x; v < kVPB; v += kThreads) {
-            if (sdf) sdf[(size_t)b * kVPB + v] = fresh ? 0.f : ld(sd + v);
-            if (w) w[(size_t)b * kVPB + v] = fresh ? 0.f : ld(sd + kVPB + v);
+            float vs = 0.f, vw = 0.f;
+            if (!fresh)
+                ld_pair(sd, v, vs, vw);
+            if (sdf) sdf[(size_t)b * kVPB + v] = vs;
+            if (w) w[(size_t)b * kVPB + v] = vw;
             if (rgb && g.rgb)
                 for (int ch = 0; ch < 3; ++ch)
                     rgb[((size_t)b * kVPB + v) * 3 + ch] = fresh ? 0.f : g.rgb[(size_t)b * 3 * kVPB + ch * kVPB + v];
@@ -932,6 +805,7 @@ struct tsdf_grid {
     void* payload;
     float* rgb;
     uint32_t* touched;
+    uint32_t* sb_bits;
     float4* ptmap;
     Ctrl* ctrl;
     Ctrl* ctrl_host;  // pinned
@@ -953,6 +827,7 @@ struct tsdf_grid {
     int* prof_kind;
     double prof_ms[TSDF_N_KERNEL_KINDS];
     uint64_t prof_n[TSDF_N_KERNEL_KINDS];
+    tsdf_tuning tune;
 };
 
 namespace {
@@ -999,11 +874,13 @@ GridView view(const tsdf_grid* g)
     v.payload = g->payload;
     v.rgb = g->rgb;
     v.touched = g->touched;
+    v.sb_bits = g->sb_bits;
     v.ctrl = g->ctrl;
     v.mask = g->p.n_buckets - 1u;
     v.n_bricks = g->p.n_bricks;
     v.vl = g->p.voxel_length;
     v.vbl = g->vbl;
+    v.rvbl = 1.0f / g->vbl;
     v.trunc = g->p.truncation;
     v.rmin = g->p.min_depth;
     v.dmax = g->p.max_ray_length;
@@ -1048,8 +925,8 @@ void make_pose(const float* E, Pose* P)
 
 int persistent_grid(const tsdf_grid* g)
 {
-    // eight resident workgroups per CU, a multiple of the 8 XCDs
-    int n = g->n_cu * 8;
+    // resident workgroups per CU (tunable), a multiple of the 8 XCDs
+    int n = g->n_cu * g->tune.integrate_wg_per_cu;
     n = (n + 7) & ~7;
     return n < 64 ? 64 : n;
 }
@@ -1151,27 +1028,38 @@ struct ProfScope {
 };
 
 template <bool kStats>
-void launch_raycast(const tsdf_grid* g, const Pose& P, float* points, float* colors, float* depth, uint8_t* ub,
+void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, float* depth, uint8_t* ub,
                     const Footprint& fp, hipStream_t s)
 {
-    const int tiles_x = (g->p.width + 15) / 16, tiles_y = (g->p.height + 15) / 16;
-    const int n_tiles = tiles_x * tiles_y;
-    const dim3 grid(((n_tiles + 7) / 8) * 8);
+    RayParams rp;
+    rp.vl = g->p.voxel_length;
+    rp.vbl = g->vbl;
+    rp.dt = g->p.truncation;
+    rp.dmax = g->p.max_ray_length;
+    rp.rvl = 1.0f / rp.vl;
+    rp.rvbl = 1.0f / rp.vbl;
+    rp.back = rp.vbl - rp.dt;
+    rp.H = g->p.height;
+    rp.W = g->p.width;
+    rp.xcd_map = g->tune.raycast_xcd_map;
+    rp.use_sb = g->tune.raycast_superblocks;
+    const int nt = ((rp.W + 15) / 16) * ((rp.H + 15) / 16);
+    const dim3 grid(((nt + 7) / 8) * 8);
     const bool color = g->p.with_color != 0;
     if (g->p.storage == TSDF_STORAGE_F16) {
         if (color)
-            hipLaunchKernelGGL((k_raycast<__half, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
-                               colors, depth, ub, tiles_x, n_tiles, fp);
+            hipLaunchKernelGGL((k_raycast<__half, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp, points,
+                               colors, depth, ub, fp);
         else
-            hipLaunchKernelGGL((k_raycast<__half, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
-                               colors, depth, ub, tiles_x, n_tiles, fp);
+            hipLaunchKernelGGL((k_raycast<__half, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp,
+                               points, colors, depth, ub, fp);
     } else {
         if (color)
-            hipLaunchKernelGGL((k_raycast<float, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
-                               colors, depth, ub, tiles_x, n_tiles, fp);
+            hipLaunchKernelGGL((k_raycast<float, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp, points,
+                               colors, depth, ub, fp);
         else
-            hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, points,
-                               colors, depth, ub, tiles_x, n_tiles, fp);
+            hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp, points,
+                               colors, depth, ub, fp);
     }
 }
 
@@ -1277,6 +1165,7 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
     if (hipGetDeviceProperties(&prop, p->device) == hipSuccess)
         ncu = prop.multiProcessorCount;
     g->n_cu = ncu;
+    tsdf_default_tuning(&g->tune);
 
     const size_t nb = p->n_buckets, nbr = p->n_bricks;
     const size_t tsz = p->storage == TSDF_STORAGE_F16 ? 2 : 4;
@@ -1295,6 +1184,7 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
     if (p->with_color)
         alloc((void**)&g->rgb, nbr * 3 * kVPB * sizeof(float));
     alloc((void**)&g->touched, nb * sizeof(uint32_t));
+    alloc((void**)&g->sb_bits, kSbWords * sizeof(uint32_t));
     alloc((void**)&g->ptmap, (size_t)p->height * p->width * sizeof(float4));
     alloc((void**)&g->ctrl, sizeof(Ctrl));
     if (st == TSDF_OK) {
@@ -1328,6 +1218,7 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->payload);
     (void)hipFree(g->rgb);
     (void)hipFree(g->touched);
+    (void)hipFree(g->sb_bits);
     (void)hipFree(g->ptmap);
     (void)hipFree(g->ctrl);
     (void)hipFree(g->op_mask);
@@ -1525,7 +1416,7 @@ tsdf_status tsdf_raycast(tsdf_grid* g, const float E[16], float* points, float* 
     memset(&fp, 0, sizeof(fp));
     {
         ProfScope ps(g, s, TSDF_K_RAYCAST);
-        launch_raycast<false>(g, P, points, g->p.with_color ? colors : nullptr, depth, ub_flags, fp, s);
+        launch_raycast<false>(g, P, points, colors, depth, ub_flags, fp, s);
     }
     HIPC(hipGetLastError());
     return TSDF_OK;
@@ -1572,6 +1463,35 @@ tsdf_status tsdf_raycast_footprint(tsdf_grid* g, const float E[16], uint64_t* di
                                        std::to_string(c[1]) + " refused");
     *distinct_voxels = c[2];
     *distinct_keys = c[0];
+    return TSDF_OK;
+}
+
+void tsdf_default_tuning(tsdf_tuning* t)
+{
+    if (!t)
+        return;
+    t->raycast_xcd_map = 1;
+    t->raycast_superblocks = 0;
+    t->integrate_wg_per_cu = 4;
+}
+
+tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
+{
+    if (!g || !t)
+        return fail(TSDF_BAD_ARG, "NULL grid or tuning");
+    if (t->raycast_xcd_map < 0 || t->raycast_xcd_map > 1 || t->raycast_superblocks < 0 ||
+        t->raycast_superblocks > 1 || t->integrate_wg_per_cu < 1 ||
+        t->integrate_wg_per_cu > 32)
+        return fail(TSDF_BAD_ARG, "workgroups per CU must lie in [1, 32]");
+    g->tune = *t;
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_get_tuning(const tsdf_grid* g, tsdf_tuning* t)
+{
+    if (!g || !t)
+        return fail(TSDF_BAD_ARG, "NULL grid or tuning");
+    *t = g->tune;
     return TSDF_OK;
 }
 

@@ -68,6 +68,33 @@ __host__ __device__ __forceinline__ int vin_index(float p, int vb, float vbl, fl
     return (q >= 0) ? f2i(q / vl) : f2i(q / vl - 1.0f);
 }
 
+// Fast path of the block / voxel index: with rd = 1 / d rounded,
+// q = a * rd is within |a/d| * 2^-22 of RN(a / d).  When q is farther than
+// max(|q|, 1) * 2^-19 from every integer (ok = true), both RN(a / d) and
+// RN(RN(a / d) - 1) lie strictly between the same two integers as q, so the
+// reference's truncation (positive branch) and its "- 1 then truncate"
+// (negative branch) both equal floor(q).  Otherwise (near an integer,
+// |q| >= 2^19, NaN / Inf) the caller must run the reference arithmetic.
+__host__ __device__ __forceinline__ int div_fast(float a, float rd, bool& ok)
+{
+    const float q = a * rd;
+    const float f = floorf(q);
+    const float fr = q - f;
+    const float m = fmaxf(fabsf(q), 1.0f) * 0x1p-19f;
+    ok = fr > m && fr < 1.0f - m;
+    return ok ? (int)f : 0;
+}
+
+// Same value as vb_index(a, d) (and, with a = p - vbl * vb, as vin_index).
+__host__ __device__ __forceinline__ int div_index(float a, float d, float rd)
+{
+    bool ok;
+    const int f = div_fast(a, rd, ok);
+    if (ok)
+        return f;
+    return (a >= 0) ? f2i(a / d) : f2i(a / d - 1.0f);
+}
+
 // getVoxelMidPoint per axis, GPUVoxelGrid.h:275-278: the int * float product
 // is float, the (v + 0.5) * vl term and the sum are double, rounded to float.
 __host__ __device__ __forceinline__ float voxel_mid(int v, int vb, float vbl, float vl)

@@ -229,6 +229,18 @@ class GPUVoxelGrid:
               "tsdf_raycast_footprint")
         return {"u_vox": int(nv.value), "u_blk": int(nk.value)}
 
+    def tuning(self) -> dict:
+        t = _lib.TsdfTuning()
+        check(self._L.tsdf_get_tuning(self._h, C.byref(t)), "tsdf_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = _lib.TsdfTuning()
+        check(self._L.tsdf_get_tuning(self._h, C.byref(t)), "tsdf_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(self._L.tsdf_set_tuning(self._h, C.byref(t)), "tsdf_set_tuning")
+
     def profile(self, enable=True):
         check(self._L.tsdf_profile(self._h, int(bool(enable))), "tsdf_profile")
 
