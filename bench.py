@@ -48,12 +48,18 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=600)
-    ap.add_argument("--cpu-frames", type=int, default=24,
+    ap.add_argument("--cpu-frames", type=int, default=120,
                     help="frames of the sequence the CPU baseline processes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reintegration", action="store_true")
     ap.add_argument("--keyframes", type=int, default=64)
     ap.add_argument("--footprint-samples", type=int, default=12)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run 2 steps of the workload plus a 256 MiB calibration copy and exit "
+                         "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 counter_collection.csv files of --pmc-workload runs into "
+                         "profiles/pmc_traffic.json and profiles/pmc_traffic_room_large.txt")
     return ap.parse_args()
 
 
@@ -217,8 +223,84 @@ def reintegration(cfg, prims, poses64, keyframes, dev):
             "bit_identical": same}
 
 
+CAL_BYTES = 256 << 20   # calibration copy: 256 MiB read + 256 MiB written
+KERNEL_NAMES = {"collect": "k_collect(", "integrate": "k_integrate<", "raycast": "k_raycast<"}
+
+
+def pmc_workload(args):
+    """Two steps of the workload and one plain copy kernel of known size."""
+    import torch
+    from superscanner_amd.voxel_grid import GPUVoxelGrid
+    cfg, prims, poses64, poses, depth, _ = workload(args.frames)
+    dev = torch.device("cuda", 0)
+    depth_dev = torch.from_numpy(depth.view(np.int16)).to(dev)
+    grid = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], cfg["H"], cfg["W"],
+                        n_buckets=cfg["n_buckets"], n_bricks=1 << 21, max_weight=cfg["max_weight"])
+    grid.setIntrinsic(*cfg["intr"])
+    run = Runner(grid, depth_dev, poses, args.frames)
+    for _ in range(2):
+        run.step()
+    a = torch.ones(CAL_BYTES // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.mul(a, 1.0, out=b)  # the calibration copy: one elementwise kernel
+    torch.cuda.synchronize()
+    grid.close()
+    print("[bench] pmc workload done", file=sys.stderr, flush=True)
+
+
+def summarize_pmc(fetch_csv, write_csv):
+    """HBM bytes per launch of each kernel from FETCH_SIZE / WRITE_SIZE (KiB
+    per dispatch), each scaled so that the calibration copy reads exactly its
+    known size."""
+    import csv
+    import statistics
+
+    def load(path, counter):
+        per = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"]
+            key = next((k for k, pat in KERNEL_NAMES.items() if pat in name), None)
+            if key is None and "MulFunctor" in name:
+                key = "copy"
+            if key:
+                per.setdefault(key, []).append(float(r["Counter_Value"]) * 1024.0)
+        return {k: statistics.median(v) for k, v in per.items()}
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    f_scale, w_scale = CAL_BYTES / fetch["copy"], CAL_BYTES / write["copy"]
+    out = {"calibration": {"copy_bytes": CAL_BYTES, "fetch_scale": round(f_scale, 4),
+                           "write_scale": round(w_scale, 4)}}
+    lines = ["HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE; median over the",
+             "launches of two room_large steps), each counter scaled by the factor that makes a plain",
+             f"256 MiB copy kernel read / write exactly 256 MiB (fetch x{f_scale:.3f}, write x{w_scale:.3f}).",
+             "Commands:",
+             "  rocprofv3 --pmc FETCH_SIZE --kernel-trace --output-format csv -d <dir> -o p -- python3 bench.py "
+             "--pmc-workload",
+             "  rocprofv3 --pmc WRITE_SIZE --kernel-trace --output-format csv -d <dir> -o p -- python3 bench.py "
+             "--pmc-workload",
+             "  python3 bench.py --summarize-pmc <fetch dir>/p_counter_collection.csv "
+             "<write dir>/p_counter_collection.csv", ""]
+    for k in KERNEL_NAMES:
+        if k in fetch and k in write:
+            fb, wb = fetch[k] * f_scale, write[k] * w_scale
+            out[k] = {"fetch_bytes_per_launch": int(fb), "write_bytes_per_launch": int(wb),
+                      "hbm_bytes_per_launch": int(fb + wb)}
+            lines.append(f"{k:10s} fetch {fb / 1e6:9.2f} MB  write {wb / 1e6:9.2f} MB  total {(fb + wb) / 1e6:9.2f} MB")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "pmc_traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    with open(os.path.join(ROOT, "profiles", "pmc_traffic_room_large.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     args = parse()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build has no "
               "multi-GPU path)", file=sys.stderr)
@@ -226,6 +308,9 @@ def main():
     import torch
     from superscanner_amd.voxel_grid import GPUVoxelGrid
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    if args.pmc_workload:
+        pmc_workload(args)
+        return
     dev = torch.device("cuda", 0)
     frames = args.frames
     cfg, prims, poses64, poses, depth, gen_s = workload(frames)
