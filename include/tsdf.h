@@ -158,6 +158,12 @@ tsdf_status tsdf_reintegrate_batch(tsdf_grid *g, int32_t n, const uint16_t *cons
                                    int32_t pitch, const uint8_t *const *rgbs,
                                    const float *E_old, const float *E_new, void *stream);
 
+/* Allocates the device buffers of a batch of up to n_ops ops (per-slot op
+ * masks and one point map per op of a pass, at most 256 maps), so that later
+ * batches of at most n_ops ops allocate nothing.  Without it the first batch
+ * allocates.  No reference counterpart (the reference has no batch). */
+tsdf_status tsdf_reserve_batch(tsdf_grid *g, int32_t n_ops);
+
 /* GPUVoxelGrid::raycast(space, extr), GPUVoxelGrid.cpp:139-147 with
  * raycastingTag :561-715.  points: device [H][W][3] float, world space,
  * (0,0,0) on a miss (rayPcd).  colors: device [H][W][3] float in [0,1] or

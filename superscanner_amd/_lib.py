@@ -30,7 +30,7 @@ RAY_UB_MISSING_OUT = 2
 EXPORTS = (
     "tsdf_default_params", "tsdf_create", "tsdf_destroy", "tsdf_reset",
     "tsdf_set_intrinsics", "tsdf_integrate", "tsdf_integrate_batch",
-    "tsdf_reintegrate_batch", "tsdf_raycast", "tsdf_sync", "tsdf_stats",
+    "tsdf_reintegrate_batch", "tsdf_reserve_batch", "tsdf_raycast", "tsdf_sync", "tsdf_stats",
     "tsdf_export_bricks", "tsdf_pose_prepare", "tsdf_status_string",
     "tsdf_last_error", "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
     "tsdf_default_tuning", "tsdf_get_tuning", "tsdf_set_tuning",
@@ -114,6 +114,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_integrate.argtypes = [vp, vp, C.c_int32, vp, fp, C.c_int32, vp]
     L.tsdf_integrate_batch.argtypes = [vp, C.c_int32, P(vp), C.c_int32, P(vp), fp, P(C.c_int32), vp]
     L.tsdf_reintegrate_batch.argtypes = [vp, C.c_int32, P(vp), C.c_int32, P(vp), fp, fp, vp]
+    L.tsdf_reserve_batch.argtypes = [vp, C.c_int32]
     L.tsdf_raycast.argtypes = [vp, fp, vp, vp, vp, vp, vp]
     L.tsdf_sync.argtypes = [vp, vp]
     L.tsdf_stats.argtypes = [vp, P(TsdfStats), vp]
@@ -131,7 +132,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_last_error.argtypes = []
     L.tsdf_last_error.restype = C.c_char_p
     for name in ("tsdf_create", "tsdf_reset", "tsdf_set_intrinsics", "tsdf_integrate",
-                 "tsdf_integrate_batch", "tsdf_reintegrate_batch", "tsdf_raycast",
+                 "tsdf_integrate_batch", "tsdf_reintegrate_batch", "tsdf_reserve_batch", "tsdf_raycast",
                  "tsdf_sync", "tsdf_stats", "tsdf_export_bricks", "tsdf_pose_prepare",
                  "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
                  "tsdf_get_tuning", "tsdf_set_tuning"):

@@ -12,7 +12,7 @@
 //   rgb        [n_bricks][3][512] float   optional colour, channel-planar
 //   touched    [n_buckets] u32   slots claimed by the current call
 //   ptmap      [H][W] float4     world point of every pixel of the current
-//                                frame, w = 1 where camera z > rmin
+//                                frame; x = NaN where camera z <= rmin
 //
 // Kernels per integrate call: k_collect (allocation, one pass over pixels)
 // and k_integrate (one wavefront per touched brick).  The brick count never
@@ -253,7 +253,9 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
             continue;
         const float d = (float)op.depth[(size_t)i * op.pitch + j] / 1000.0f;
         const f3 p = get_point(g.cam, i, j, d);
-        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+        // pixels without a measurement (z <= rmin) carry x = NaN: any sdf
+        // computed from them is NaN and fails the |sdf| < trunc test
+        float4 o = make_float4(__builtin_nanf(""), 0.f, 0.f, 0.f);
         if (p.z > g.rmin) {
             ++nvalid;
             const f3 pt = xform(p, op.pose.E);
@@ -337,71 +339,162 @@ __global__ __launch_bounds__(kThreads) void k_collect_batch(GridView g, const Op
 // GPUVoxelGrid.cpp:345-469) on register-resident values
 // ---------------------------------------------------------------------------
 
-// The running-average update of one voxel by one measurement sdf (and pixel
-// colour px), GPUVoxelGrid.cpp:360-361, 384-391, 412-420, 450-464.
-template <bool kColor>
-__device__ __forceinline__ void apply_update(const GridView& g, const OpDesc& op, float sdf, const float* px,
-                                             float& s, float& w, float* c)
+// Row-major pixel of the voxel centre with camera coordinates c, or -1 when
+// it falls outside the image (GPUVoxelGrid.cpp:347-352).  The reciprocal
+// comes from v_rcp_f32 and is certified before use (rcp_certified), so the
+// result rests on IEEE fma / mul / add / floor only, not on the accuracy of
+// the hardware reciprocal.
+__device__ __forceinline__ int project_voxel(const GridView& g, float cx, float cy, float cz)
 {
-    if (op.sign > 0) {
-        s = (s * w + sdf) / (w + 1.0f);
-        if (kColor && op.rgb) {
-            c[0] = (c[0] * w + px[0]) / (w + 1.0f);
-            c[1] = (c[1] * w + px[1]) / (w + 1.0f);
-            c[2] = (c[2] * w + px[2]) / (w + 1.0f);
-        }
-        // Weight cap: project semantics, inactive at max_weight == 0 (the
-        // reference never caps, GPUVoxelGrid.h:14,62).
-        w = (g.wmax > 0.0f && w >= g.wmax) ? g.wmax : w + 1.0f;
-    } else {
-        if (w != 1.0f) {
-            s = (s * w - sdf) / (w - 1.0f);
-            if (kColor && op.rgb) {
-                c[0] = (c[0] * w - px[0]) / (w - 1.0f);
-                c[1] = (c[1] * w - px[1]) / (w - 1.0f);
-                c[2] = (c[2] * w - px[2]) / (w - 1.0f);
+    const float r = __builtin_amdgcn_rcpf(cz);
+    const float ny = cy * g.cam.fy, nx = cx * g.cam.fx;
+    int i, j;
+    const bool ok = rcp_certified(r, cz) & pix_axis_fast(ny, r, g.cam.ci, i) & pix_axis_fast(nx, r, g.cam.cj, j);
+    if (!ok) {
+        i = pix_axis_ref(ny, cz, g.cam.ci);
+        j = pix_axis_ref(nx, cz, g.cam.cj);
+    }
+    return ((unsigned)i < (unsigned)g.H && (unsigned)j < (unsigned)g.W) ? i * g.W + j : -1;
+}
+
+// The voxels (x, y, 0..7) of one lane, held in registers across ops.
+template <bool kColor>
+struct LaneVoxels {
+    float s[kVPE], w[kVPE];
+    float c[kColor ? kVPE : 1][3];
+};
+
+// One integrate (kSign = +1) or deintegrate (kSign = -1) op applied to a
+// lane's eight voxels, in the reference's arithmetic order:
+//   c = transformPoint(vm, E_inv), (row, col) = getCoordinate(c)
+//   vpt = inputPcd(row, col) (from ptmap; x is NaN where z <= rmin, which
+//   makes sdf NaN and fails the |sdf| < trunc test exactly as the
+//   reference's `vpt.z > rmin` test does)
+//   sdf = (vpt - vm) . Ez; update when |sdf| < trunc:
+//     +1: s = (s w + sdf) / (w + 1), w = w + 1 (capped at max_weight, project
+//         semantics: the reference never caps, GPUVoxelGrid.h:14,62)
+//     -1: w != 1: s = (s w - sdf) / (w - 1), w = w - 1; else s = w = 0
+// Returns a bit per voxel that was updated.
+template <bool kColor, int kSign>
+__device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc& op, float vmx, float vmy,
+                                                 const float (&vmz)[kVPE], LaneVoxels<kColor>& v)
+{
+    const float* Ei = op.pose.Ei;
+    // transformPoint with the z-independent part hoisted; the association
+    // ((x*E0 + y*E1) + z*E2) + E3 is the reference's
+    const float a0 = vmx * Ei[0] + vmy * Ei[1];
+    const float a1 = vmx * Ei[4] + vmy * Ei[5];
+    const float a2 = vmx * Ei[8] + vmy * Ei[9];
+    int pix[kVPE];
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z)
+        pix[z] = project_voxel(g, a0 + vmz[z] * Ei[2] + Ei[3], a1 + vmz[z] * Ei[6] + Ei[7],
+                               a2 + vmz[z] * Ei[10] + Ei[11]);
+    float pt[kVPE][3];
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z) {
+        const float* q = (const float*)(op.ptmap + (pix[z] >= 0 ? pix[z] : 0));
+        pt[z][0] = q[0];
+        pt[z][1] = q[1];
+        pt[z][2] = q[2];
+    }
+    const float ez0 = op.pose.Ez[0], ez1 = op.pose.Ez[1], ez2 = op.pose.Ez[2];
+    const bool use_rgb = kColor && op.rgb != nullptr;
+    unsigned int upd = 0;
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z) {
+        const float sdf = (pt[z][0] - vmx) * ez0 + (pt[z][1] - vmy) * ez1 + (pt[z][2] - vmz[z]) * ez2;
+        if (pix[z] >= 0 && fabsf(sdf) < g.trunc) {
+            upd |= 1u << z;
+            const float s = v.s[z], w = v.w[z];
+            float px[3] = {0.f, 0.f, 0.f};
+            if (use_rgb) {
+                const uint8_t* q = op.rgb + (size_t)pix[z] * 3;
+                px[0] = (float)q[0];
+                px[1] = (float)q[1];
+                px[2] = (float)q[2];
             }
-            w = w - 1.0f;
-        } else {
-            w = 0.0f;
-            s = 0.0f;
-            if (kColor && op.rgb)
-                c[0] = c[1] = c[2] = 0.0f;
+            if (kSign > 0) {
+                const float d = w + 1.0f;
+                v.s[z] = (s * w + sdf) / d;
+                if (use_rgb)
+                    for (int ch = 0; ch < 3; ++ch)
+                        v.c[z][ch] = (v.c[z][ch] * w + px[ch]) / d;
+                v.w[z] = (g.wmax > 0.0f && w >= g.wmax) ? g.wmax : d;
+            } else {
+                const float d = w - 1.0f;
+                const bool one = w == 1.0f;
+                v.s[z] = one ? 0.0f : (s * w - sdf) / d;
+                if (use_rgb)
+                    for (int ch = 0; ch < 3; ++ch)
+                        v.c[z][ch] = one ? 0.0f : (v.c[z][ch] * w - px[ch]) / d;
+                v.w[z] = one ? 0.0f : d;
+            }
+        }
+    }
+    return upd;
+}
+
+template <typename T, bool kColor>
+__device__ __forceinline__ void load_lane(const GridView& g, int id, int lane, bool fresh, LaneVoxels<kColor>& v)
+{
+    const T* sd = (const T*)g.payload + (size_t)id * (2 * kVPB);
+    const float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z) {
+        v.s[z] = v.w[z] = 0.f;
+        if (kColor)
+            v.c[z][0] = v.c[z][1] = v.c[z][2] = 0.f;
+        if (!fresh) {
+            ld_pair(sd, lane + 64 * z, v.s[z], v.w[z]);
+            if (kColor) {
+                v.c[z][0] = cr[lane + 64 * z];
+                v.c[z][1] = cr[lane + 64 * z + kVPB];
+                v.c[z][2] = cr[lane + 64 * z + 2 * kVPB];
+            }
         }
     }
 }
 
-template <bool kColor>
-__device__ __forceinline__ bool update_voxel(const GridView& g, const OpDesc& op, float vmx, float vmy,
-                                             float vmz, float a0, float a1, float a2, float& s, float& w,
-                                             float* c)
+// Writes back the voxels flagged in `dirty` (all of them for a fresh brick,
+// whose payload memory was never initialised).
+template <typename T, bool kColor>
+__device__ __forceinline__ void store_lane(const GridView& g, int id, int lane, unsigned int dirty,
+                                           const LaneVoxels<kColor>& v)
 {
-    const float* Ei = op.pose.Ei;
-    // transformPoint(vm, E_inv) with the z-independent part hoisted:
-    // ((x*E0 + y*E1) + z*E2) + E3 keeps the reference's association.
-    const float cx = a0 + vmz * Ei[2] + Ei[3];
-    const float cy = a1 + vmz * Ei[6] + Ei[7];
-    const float cz = a2 + vmz * Ei[10] + Ei[11];
-    const float row = cy * g.cam.fy / cz + g.cam.ci;  // getCoordinate, GPUVoxelGrid.h:234-237
-    const float col = cx * g.cam.fx / cz + g.cam.cj;
-    const int cori = f2i(row), corj = f2i(col);
-    if (!(cori < g.H && corj < g.W && cori >= 0 && corj >= 0))
-        return false;
-    const float4 pt = op.ptmap[(size_t)cori * g.W + corj];
-    if (pt.w == 0.0f)  // inputPcd(cori, corj).z <= rmin
-        return false;
-    const float sdf = (pt.x - vmx) * op.pose.Ez[0] + (pt.y - vmy) * op.pose.Ez[1] + (pt.z - vmz) * op.pose.Ez[2];
-    if (!(fabsf(sdf) < g.trunc))
-        return false;
-    float px[3] = {0.f, 0.f, 0.f};
-    if (kColor && op.rgb) {
-        const uint8_t* q = op.rgb + ((size_t)cori * g.W + corj) * 3;
-        px[0] = (float)q[0];
-        px[1] = (float)q[1];
-        px[2] = (float)q[2];
+    T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
+    float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z) {
+        if (!((dirty >> z) & 1u))
+            continue;
+        const int k = lane + 64 * z;
+        st_pair(sd, k, v.s[z], v.w[z]);
+        if (kColor) {
+            cr[k] = v.c[z][0];
+            cr[k + kVPB] = v.c[z][1];
+            cr[k + 2 * kVPB] = v.c[z][2];
+        }
     }
-    apply_update<kColor>(g, op, sdf, px, s, w, c);
-    return true;
+}
+
+// Voxel-centre coordinates of a lane: getVoxelMidPoint with the reference's
+// double promotion of (k + 0.5) * vl (GPUVoxelGrid.cpp:501,521).
+struct LaneMid {
+    float x, y, z[kVPE];
+};
+
+__device__ __forceinline__ LaneMid lane_mid(const GridView& g, const Offsets& off, int bx, int by, int bz,
+                                            int lane)
+{
+    LaneMid m;
+    const float pzx = g.vbl * (float)bx, pzy = g.vbl * (float)by, pzz = g.vbl * (float)bz;
+    m.x = (float)((double)pzx + off.v[lane & 7]);
+    m.y = (float)((double)pzy + off.v[lane >> 3]);
+#pragma unroll
+    for (int z = 0; z < kVPE; ++z)
+        m.z[z] = (float)((double)pzz + off.v[z]);
+    return m;
 }
 
 // Which touched-list entries a wavefront handles.  Workgroups are dealt to
@@ -439,13 +532,8 @@ __device__ __forceinline__ WaveRange wave_range(unsigned int count)
 // ---------------------------------------------------------------------------
 constexpr int kMetaChunk = 16;
 
-struct BrickMeta {
-    int id;  // brick id, or -1 when the slot's allocation failed
-    int x, y, z;
-    bool fresh;
-};
-
-// Resolves touched[t0 .. t0 + kMetaChunk) into LDS (threads 0..63).
+// Resolves touched[t0 .. t1) (at most kMetaChunk entries) into LDS:
+// (id | fresh << 31, x, y, z), id = -1 when the slot's allocation failed.
 __device__ __forceinline__ void load_meta(const GridView& g, unsigned int t0, unsigned int t1, int4* meta)
 {
     if (threadIdx.x < kMetaChunk) {
@@ -464,89 +552,7 @@ __device__ __forceinline__ void load_meta(const GridView& g, unsigned int t0, un
     }
 }
 
-template <typename T, bool kColor>
-__device__ __forceinline__ unsigned int integrate_brick(const GridView& g, const OpDesc& op, const Offsets& off,
-                                                        int4 m, int lane)
-{
-    const bool fresh = m.x < 0;  // bit 31: allocated by this call, never written
-    const int id = m.x & 0x7FFFFFFF;
-    const int x = lane & 7, y = lane >> 3;
-    T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
-    float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
-    float s[kVPE], w[kVPE], col[kVPE][3];
-#pragma unroll
-    for (int z = 0; z < kVPE; ++z) {
-        s[z] = w[z] = 0.f;
-        col[z][0] = col[z][1] = col[z][2] = 0.f;
-        if (!fresh) {
-            ld_pair(sd, lane + 64 * z, s[z], w[z]);
-            if (kColor) {
-                col[z][0] = cr[lane + 64 * z];
-                col[z][1] = cr[lane + 64 * z + kVPB];
-                col[z][2] = cr[lane + 64 * z + 2 * kVPB];
-            }
-        }
-    }
-    const float pzx = g.vbl * (float)m.y, pzy = g.vbl * (float)m.z, pzz = g.vbl * (float)m.w;
-    const float vmx = (float)((double)pzx + off.v[x]);
-    const float vmy = (float)((double)pzy + off.v[y]);
-    const float* Ei = op.pose.Ei;
-    const float a0 = vmx * Ei[0] + vmy * Ei[1];
-    const float a1 = vmx * Ei[4] + vmy * Ei[5];
-    const float a2 = vmx * Ei[8] + vmy * Ei[9];
-    // projection of every voxel (transformPoint(vm, E_inv) + getCoordinate,
-    // GPUVoxelGrid.cpp:347-350), then all gathers of inputPcd at once
-    float vmz[kVPE];
-    int pix[kVPE];
-#pragma unroll
-    for (int z = 0; z < kVPE; ++z) {
-        vmz[z] = (float)((double)pzz + off.v[z]);
-        const float cx = a0 + vmz[z] * Ei[2] + Ei[3];
-        const float cy = a1 + vmz[z] * Ei[6] + Ei[7];
-        const float cz = a2 + vmz[z] * Ei[10] + Ei[11];
-        const float row = cy * g.cam.fy / cz + g.cam.ci;
-        const float colf = cx * g.cam.fx / cz + g.cam.cj;
-        const int cori = f2i(row), corj = f2i(colf);
-        pix[z] = (cori < g.H && corj < g.W && cori >= 0 && corj >= 0) ? cori * g.W + corj : -1;
-    }
-    float4 pt[kVPE];
-#pragma unroll
-    for (int z = 0; z < kVPE; ++z)
-        pt[z] = op.ptmap[pix[z] >= 0 ? pix[z] : 0];
-    unsigned int nupd = 0;
-#pragma unroll
-    for (int z = 0; z < kVPE; ++z) {
-        const int v = lane + 64 * z;
-        bool upd = false;
-        if (pix[z] >= 0 && pt[z].w != 0.0f) {  // inputPcd(cori, corj).z > rmin
-            const float sdf = (pt[z].x - vmx) * op.pose.Ez[0] + (pt[z].y - vmy) * op.pose.Ez[1] +
-                              (pt[z].z - vmz[z]) * op.pose.Ez[2];
-            if (fabsf(sdf) < g.trunc) {
-                upd = true;
-                float px[3] = {0.f, 0.f, 0.f};
-                if (kColor && op.rgb) {
-                    const uint8_t* q = op.rgb + (size_t)pix[z] * 3;
-                    px[0] = (float)q[0];
-                    px[1] = (float)q[1];
-                    px[2] = (float)q[2];
-                }
-                apply_update<kColor>(g, op, sdf, px, s[z], w[z], col[z]);
-            }
-        }
-        if (upd || fresh) {
-            st_pair(sd, v, s[z], w[z]);
-            if (kColor) {
-                cr[v] = col[z][0];
-                cr[v + kVPB] = col[z][1];
-                cr[v + 2 * kVPB] = col[z][2];
-            }
-        }
-        nupd += upd ? 1u : 0u;
-    }
-    return nupd;
-}
-
-template <typename T, bool kColor>
+template <typename T, bool kColor, int kSign>
 __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, int par, Offsets off)
 {
     __shared__ int4 meta[kMetaChunk];
@@ -568,8 +574,16 @@ __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, i
         __syncthreads();
         for (int e = wave; e < kMetaChunk; e += kThreads / 64) {
             const int4 m = meta[e];
-            if (m.x != -1)
-                nupd += integrate_brick<T, kColor>(g, op, off, m, lane);
+            if (m.x == -1)
+                continue;
+            const bool fresh = m.x < 0;  // bit 31: allocated by this call
+            const int id = m.x & 0x7FFFFFFF;
+            LaneVoxels<kColor> v;
+            load_lane<T, kColor>(g, id, lane, fresh, v);
+            const LaneMid mid = lane_mid(g, off, m.y, m.z, m.w, lane);
+            const unsigned int upd = apply_op<kColor, kSign>(g, op, mid.x, mid.y, mid.z, v);
+            nupd += __popc(upd);
+            store_lane<T, kColor>(g, id, lane, fresh ? 0xFFu : upd, v);
         }
     }
     nupd = wave_sum(nupd);
@@ -592,7 +606,6 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
         g.ctrl->total_touched += count;
     }
     const int lane = threadIdx.x & 63;
-    const int x = lane & 7, y = lane >> 3;
     unsigned int nupd = 0;
     const WaveRange wr = wave_range(count);
     for (unsigned int t = wr.begin; t < wr.end; t += wr.stride) {
@@ -610,70 +623,28 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
         const bool fresh = (c.w & 1) != 0;
         if (fresh && lane == 0)
             g.coords[id].w = c.w & ~1;
-        const float pzx = g.vbl * (float)c.x, pzy = g.vbl * (float)c.y, pzz = g.vbl * (float)c.z;
-        const float vmx = (float)((double)pzx + off.v[x]);
-        const float vmy = (float)((double)pzy + off.v[y]);
-        T* sd = (T*)g.payload + (size_t)id * (2 * kVPB);
-        float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
-        float s[kVPE], w[kVPE], col[kVPE][3];
-        float vmz[kVPE];
-        bool dirty[kVPE];
-#pragma unroll
-        for (int z = 0; z < kVPE; ++z) {
-            const int v = lane + 64 * z;
-            vmz[z] = (float)((double)pzz + off.v[z]);
-            dirty[z] = fresh;
-            s[z] = 0.f;
-            w[z] = 0.f;
-            if (!fresh)
-                ld_pair(sd, v, s[z], w[z]);
-            col[z][0] = col[z][1] = col[z][2] = 0.f;
-            if (kColor && !fresh) {
-                col[z][0] = cr[v];
-                col[z][1] = cr[v + kVPB];
-                col[z][2] = cr[v + 2 * kVPB];
-            }
-        }
+        LaneVoxels<kColor> v;
+        load_lane<T, kColor>(g, (int)id, lane, fresh, v);
+        const LaneMid mid = lane_mid(g, off, c.x, c.y, c.z, lane);
+        unsigned int dirty = fresh ? 0xFFu : 0u;
         for (int k = 0; k < kMaskWords; ++k) {
             uint32_t bits = words[k];
             while (bits) {
                 const int b = __ffs(bits) - 1;
                 bits &= bits - 1u;
                 const OpDesc& op = ops[k * 32 + b];
-                const float* Ei = op.pose.Ei;
-                const float a0 = vmx * Ei[0] + vmy * Ei[1];
-                const float a1 = vmx * Ei[4] + vmy * Ei[5];
-                const float a2 = vmx * Ei[8] + vmy * Ei[9];
-#pragma unroll
-                for (int z = 0; z < kVPE; ++z) {
-                    const bool u = update_voxel<kColor>(g, op, vmx, vmy, vmz[z], a0, a1, a2, s[z], w[z], col[z]);
-                    dirty[z] = dirty[z] || u;
-                    nupd += u ? 1u : 0u;
-                }
+                const unsigned int u = op.sign > 0 ? apply_op<kColor, 1>(g, op, mid.x, mid.y, mid.z, v)
+                                                   : apply_op<kColor, -1>(g, op, mid.x, mid.y, mid.z, v);
+                dirty |= u;
+                nupd += __popc(u);
             }
         }
-#pragma unroll
-        for (int z = 0; z < kVPE; ++z) {
-            if (!dirty[z])
-                continue;
-            const int v = lane + 64 * z;
-            st_pair(sd, v, s[z], w[z]);
-            if (kColor) {
-                cr[v] = col[z][0];
-                cr[v + kVPB] = col[z][1];
-                cr[v + 2 * kVPB] = col[z][2];
-            }
-        }
+        store_lane<T, kColor>(g, (int)id, lane, dirty, v);
     }
     nupd = wave_sum(nupd);
     if (lane == 0 && nupd)
         atomicAdd(&g.ctrl->total_updated, (unsigned long long)nupd);
 }
-
-// ---------------------------------------------------------------------------
-// k_raycast: raycastingTag, GPUVoxelGrid.cpp:561-715.  One lane per pixel,
-// 8 x 8 pixels per wavefront, 16 x 16 per workgroup.
-// ---------------------------------------------------------------------------
 
 // ---------------------------------------------------------------------------
 // k_raycast: raycastingTag, GPUVoxelGrid.cpp:561-715 (march in tsdf_march.hpp).
@@ -814,6 +785,7 @@ struct tsdf_grid {
     float4* ptmap_batch;
     OpDesc* ops_dev;
     OpDesc* ops_host;  // pinned
+    int batch_cap;     // point maps allocated in ptmap_batch
     // raycast footprint buffers (allocated on first tsdf_raycast_footprint)
     uint32_t* fp_bits;
     unsigned long long* fp_keys;
@@ -935,10 +907,15 @@ template <typename T>
 void launch_integrate(const tsdf_grid* g, const OpDesc& op, int par, hipStream_t s, bool color)
 {
     const int grid = persistent_grid(g);
-    if (color)
-        hipLaunchKernelGGL((k_integrate<T, true>), dim3(grid), dim3(kThreads), 0, s, view(g), op, par, g->off);
+    const GridView v = view(g);
+    if (color && op.sign > 0)
+        hipLaunchKernelGGL((k_integrate<T, true, 1>), dim3(grid), dim3(kThreads), 0, s, v, op, par, g->off);
+    else if (color)
+        hipLaunchKernelGGL((k_integrate<T, true, -1>), dim3(grid), dim3(kThreads), 0, s, v, op, par, g->off);
+    else if (op.sign > 0)
+        hipLaunchKernelGGL((k_integrate<T, false, 1>), dim3(grid), dim3(kThreads), 0, s, v, op, par, g->off);
     else
-        hipLaunchKernelGGL((k_integrate<T, false>), dim3(grid), dim3(kThreads), 0, s, view(g), op, par, g->off);
+        hipLaunchKernelGGL((k_integrate<T, false, -1>), dim3(grid), dim3(kThreads), 0, s, v, op, par, g->off);
 }
 
 template <typename T>
@@ -962,16 +939,27 @@ tsdf_status check_image_args(const tsdf_grid* g, const uint16_t* depth, int32_t 
     return TSDF_OK;
 }
 
-tsdf_status ensure_batch_buffers(tsdf_grid* g)
+// Device buffers of a batch of n_ops ops: the per-slot op masks and op
+// descriptors (once), and one point map per op of a pass (grown to
+// min(n_ops, kMaxOps) maps; a batch never allocates once that is reserved).
+tsdf_status ensure_batch_buffers(tsdf_grid* g, int n_ops)
 {
-    if (g->op_mask)
-        return TSDF_OK;
-    const size_t nb = g->p.n_buckets;
-    HIPC(hipMalloc(&g->op_mask, nb * kMaskWords * sizeof(uint32_t)));
-    HIPC(hipMemset(g->op_mask, 0, nb * kMaskWords * sizeof(uint32_t)));
-    HIPC(hipMalloc(&g->ptmap_batch, (size_t)kMaxOps * g->p.height * g->p.width * sizeof(float4)));
-    HIPC(hipMalloc(&g->ops_dev, kMaxOps * sizeof(OpDesc)));
-    HIPC(hipHostMalloc(&g->ops_host, kMaxOps * sizeof(OpDesc), hipHostMallocDefault));
+    if (!g->op_mask) {
+        const size_t nb = g->p.n_buckets;
+        HIPC(hipMalloc(&g->op_mask, nb * kMaskWords * sizeof(uint32_t)));
+        HIPC(hipMemset(g->op_mask, 0, nb * kMaskWords * sizeof(uint32_t)));
+        HIPC(hipMalloc(&g->ops_dev, kMaxOps * sizeof(OpDesc)));
+        HIPC(hipHostMalloc(&g->ops_host, kMaxOps * sizeof(OpDesc), hipHostMallocDefault));
+    }
+    const int want = n_ops < kMaxOps ? n_ops : kMaxOps;
+    if (g->batch_cap < want) {
+        HIPC(hipDeviceSynchronize());  // a previous pass may still read the old maps
+        (void)hipFree(g->ptmap_batch);
+        g->ptmap_batch = nullptr;
+        g->batch_cap = 0;
+        HIPC(hipMalloc(&g->ptmap_batch, (size_t)want * g->p.height * g->p.width * sizeof(float4)));
+        g->batch_cap = want;
+    }
     HIPC(hipDeviceSynchronize());
     return TSDF_OK;
 }
@@ -1332,7 +1320,7 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
     if (n_ops == 0)
         return TSDF_OK;
     DeviceScope ds(g->p.device);
-    tsdf_status bs = ensure_batch_buffers(g);
+    tsdf_status bs = ensure_batch_buffers(g, n_ops);
     if (bs != TSDF_OK)
         return bs;
     hipStream_t s = (hipStream_t)stream;
@@ -1370,6 +1358,16 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
         HIPC(hipGetLastError());
     }
     return TSDF_OK;
+}
+
+tsdf_status tsdf_reserve_batch(tsdf_grid* g, int32_t n_ops)
+{
+    if (!g || n_ops < 0)
+        return fail(TSDF_BAD_ARG, "NULL handle or negative op count");
+    if (n_ops == 0)
+        return TSDF_OK;
+    DeviceScope ds(g->p.device);
+    return ensure_batch_buffers(g, n_ops);
 }
 
 tsdf_status tsdf_reintegrate_batch(tsdf_grid* g, int32_t n, const uint16_t* const* depths, int32_t pitch,

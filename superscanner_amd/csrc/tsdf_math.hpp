@@ -95,6 +95,45 @@ __host__ __device__ __forceinline__ int div_index(float a, float d, float rd)
     return (a >= 0) ? f2i(a / d) : f2i(a / d - 1.0f);
 }
 
+// Certifies an approximate reciprocal r of z: e = RN(r z - 1) (one fma)
+// with |e| <= 2^-22 bounds the relative error of r by 2^-22 (1 + 2^-23),
+// whatever produced r (0, Inf, NaN and flushed denormals all fail).
+__host__ __device__ __forceinline__ bool rcp_certified(float r, float z)
+{
+    return fabsf(__builtin_fmaf(r, z, -1.0f)) <= 0x1p-22f;
+}
+
+// Pixel index along one image axis as getCoordinate (GPUVoxelGrid.h:234-237)
+// and the (int) cast of GPUVoxelGrid.cpp:348-349 give it:
+// idx = f2i(RN(RN(n / z) + c0)) with n = RN(c * f).
+//
+// Fast path with an approximate reciprocal r of z whose relative error the
+// caller has certified to be at most 2^-22 (rcp_certified):
+//   qa = RN(n * r) is within 0.75 * 2^-21 |qa| of q = RN(n / z), and
+//   ra = RN(qa + c0) is within B = 0.75 * 2^-21 |qa| + 0.5 * 2^-22 |ra| of the
+//   reference's row = RN(q + c0).
+// When ra is farther than m = 2^-21 |qa| + 2^-22 |ra| + 2^-21 from both
+// neighbouring integers (m exceeds B by more than the rounding of m, of
+// ra - floor(ra) and of 1 - m, and by more than any underflow error), row
+// and ra lie in the same open interval (k, k+1), on which truncation is
+// constant, so f2i(row) == (int)ra.  The test fails for |ra| >= 2^23, Inf and
+// NaN, and the caller then evaluates the reference expression.
+__host__ __device__ __forceinline__ bool pix_axis_fast(float n, float r, float c0, int& idx)
+{
+    const float qa = n * r;
+    const float ra = qa + c0;
+    const float fr = ra - floorf(ra);
+    const float m = __builtin_fmaf(fabsf(qa), 0x1p-21f, __builtin_fmaf(fabsf(ra), 0x1p-22f, 0x1p-21f));
+    idx = (int)ra;
+    return (fr > m) & (fr < 1.0f - m);
+}
+
+// The reference expression of the same index: f2i(RN(RN(n / z) + c0)).
+__host__ __device__ __forceinline__ int pix_axis_ref(float n, float z, float c0)
+{
+    return f2i(n / z + c0);
+}
+
 // getVoxelMidPoint per axis, GPUVoxelGrid.h:275-278: the int * float product
 // is float, the (v + 0.5) * vl term and the sum are double, rounded to float.
 __host__ __device__ __forceinline__ float voxel_mid(int v, int vb, float vbl, float vl)

@@ -211,6 +211,11 @@ class GPUVoxelGrid:
         if fence:
             self.sync(stream)
 
+    def reserve_batch(self, n_ops):
+        """Allocates the device buffers a batch of n_ops ops needs, so that
+        later batches of at most n_ops ops allocate nothing."""
+        check(self._L.tsdf_reserve_batch(self._h, int(n_ops)), "tsdf_reserve_batch")
+
     # -- state -------------------------------------------------------------
     def sync(self, stream=None):
         check(self._L.tsdf_sync(self._h, _stream_ptr(stream)), "tsdf_sync")

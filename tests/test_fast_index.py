@@ -1,6 +1,7 @@
-"""The exact fast path of the block / voxel index (tsdf_math.hpp div_index)
-agrees with the reference arithmetic on >20M random and adversarial inputs
-(built from tests/cpp/test_div_index.cpp as host code)."""
+"""The exact fast paths of the block / voxel index (tsdf_math.hpp div_index,
+tests/cpp/test_div_index.cpp) and of the integrate kernel's pixel index
+(pix_axis_fast, tests/cpp/test_pix_index.cpp) agree with the reference
+arithmetic on millions of random and adversarial inputs (host builds)."""
 import os
 import shutil
 import subprocess
@@ -21,3 +22,23 @@ def test_div_index_matches_reference(tmp_path):
     r = subprocess.run([str(exe), "20000000"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "agree with the reference arithmetic" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
+                    reason="hipcc not available")
+def test_pix_index_matches_reference(tmp_path):
+    """The integrate kernel's fast pixel index (certified reciprocal + margin
+    test) equals f2i(n / z + c0) whenever it accepts, for reciprocals with up
+    to twice the certified error, and rejects non-finite reciprocals."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    exe = tmp_path / "test_pix_index"
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+                    "-o", str(exe), os.path.join(ROOT, "tests", "cpp", "test_pix_index.cpp")],
+                   check=True, capture_output=True)
+    r = subprocess.run([str(exe), "10000000"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all equal to the reference" in r.stdout
+    # a quarter of the reciprocals carry more than the certified error and are
+    # rejected; the fast path must still carry most inputs
+    checked, accepted = [int(w) for w in r.stdout.replace(",", " ").split() if w.isdigit()][:2]
+    assert accepted > 0.6 * checked
