@@ -167,7 +167,7 @@ def cpu_baseline(cfg, poses, depth, n):
 def reintegration(cfg, prims, poses64, keyframes, dev):
     """Batched vs plain re-fusion of `keyframes` keyframes."""
     import torch
-    from superscanner_amd import synth
+    from superscanner_amd import autotune, synth
     from superscanner_amd.voxel_grid import GPUVoxelGrid
     rng = np.random.default_rng(21)
     n = len(poses64)
@@ -181,15 +181,19 @@ def reintegration(cfg, prims, poses64, keyframes, dev):
 
     def fresh():
         g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
-                         n_bricks=1 << 21, max_weight=cfg["max_weight"])
+                         n_bricks=1 << 18, max_weight=cfg["max_weight"])
         g.setIntrinsic(*cfg["intr"])
         for k in range(keyframes):
             g.integrate(None, kd[k], old[k])
         g.sync()
         return g
 
+    kds = [kd[k] for k in range(keyframes)]
+    # launch shapes of the batch picked by the autotuner on this workload
+    tuned, table = autotune.tune_reintegration(fresh, kds, old, new)
     a, b = fresh(), fresh()
     b.reserve_batch(2 * keyframes)  # set-up, like the pool itself: the timed batch allocates nothing
+    b.set_tuning(**tuned)
     torch.cuda.synchronize()
     # plain: the integrate.cpp:108-125 loop, one deIntegrate + integrate per keyframe
     t0 = time.perf_counter()
@@ -207,15 +211,16 @@ def reintegration(cfg, prims, poses64, keyframes, dev):
         c.integrate(None, kd[k], new[k], fence=True)
     t_fenced = time.perf_counter() - t0
     t0 = time.perf_counter()
-    b.reintegrate_batch([kd[k] for k in range(keyframes)], old, new)
+    b.reintegrate_batch(kds, old, new)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     # the same batch once more on a fresh grid with per-kernel events
     p = fresh()
     p.reserve_batch(2 * keyframes)
+    p.set_tuning(**tuned)
     p.profile(True)
     p.profile_read(reset=True)
-    p.reintegrate_batch([kd[k] for k in range(keyframes)], old, new)
+    p.reintegrate_batch(kds, old, new)
     kt = p.profile_read(reset=True)
     p.close()
     ea, eb = a.export_bricks(), b.export_bricks()
@@ -229,6 +234,8 @@ def reintegration(cfg, prims, poses64, keyframes, dev):
     return {"keyframes": keyframes, "plain_calls": 2 * keyframes, "ms_plain": round(t_plain * 1e3, 3),
             "ms_plain_fenced": round(t_fenced * 1e3, 3), "ms_batched": round(t_batch * 1e3, 3),
             "speedup_vs_plain": round(t_plain / t_batch, 2), "speedup_vs_fenced": round(t_fenced / t_batch, 2),
+            "tuning": tuned,
+            "autotune_ms": {f"{per}x{wg}": round(ms, 3) for per, wg, ms in table},
             "kernel_ms": {"collect_batch": round(kt["collect_batch"]["ms"], 3),
                           "integrate_batch": round(kt["integrate_batch"]["ms"], 3)},
             "bit_identical": same}

@@ -109,6 +109,8 @@ typedef struct {
     int32_t raycast_xcd_map;      /* 1: each XCD takes one band of the image    */
     int32_t raycast_superblocks;  /* 1: skip empty superblocks when certified   */
     int32_t integrate_wg_per_cu;  /* brick-update workgroups per CU            */
+    int32_t batch_ops_per_pass;   /* ops per collection + update pass of a     */
+                                  /* batch, 1..256                              */
 } tsdf_tuning;
 
 /* Fills the defaults of the reference: d_max 10 m, rmin 0.2 m, no weight

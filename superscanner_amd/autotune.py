@@ -1,0 +1,105 @@
+"""Launch-shape autotuner for the TSDF kernels (no reference counterpart: the
+reference launches fixed Kokkos MDRange policies, GPUVoxelGrid.cpp:59-143).
+
+The tunables are the fields of ``tsdf_tuning`` (include/tsdf.h):
+
+  integrate_wg_per_cu   resident brick-update workgroups per CU (occupancy
+                        of the persistent k_integrate / k_integrate_batch)
+  batch_ops_per_pass    ops per collection + update pass of a batch (the
+                        pass's point maps, ops * H * W * 16 B, should stay in
+                        the last-level cache; more ops per pass means fewer
+                        payload round trips)
+  raycast_xcd_map       XCD-banded tile order of the raycast
+  raycast_superblocks   certified empty-superblock skipping in the raycast
+
+Each tuner times candidates on the caller's own representative workload with
+HIP events on the grid's stream and returns the fastest setting together with
+the measured table.  A batch changes the map, so every reintegration
+candidate runs on a grid rebuilt by the caller's ``make_grid``.
+"""
+from __future__ import annotations
+
+import itertools
+
+import torch
+
+INTEGRATE_WG_PER_CU = (2, 3, 4, 6, 8)
+BATCH_OPS_PER_PASS = (4, 8, 16, 32, 64, 128)
+
+
+def _timed(fn, stream=None):
+    s = stream or torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b)
+
+
+def tune_reintegration(make_grid, depths, old_poses, new_poses, *, ops_per_pass=BATCH_OPS_PER_PASS,
+                       wg_per_cu=INTEGRATE_WG_PER_CU, repeats=1):
+    """Picks (batch_ops_per_pass, integrate_wg_per_cu) for
+    ``GPUVoxelGrid.reintegrate_batch(depths, old_poses, new_poses)``.
+
+    make_grid() must return a grid in the state the batch starts from.
+    Returns (best tuning dict, [(ops_per_pass, wg_per_cu, ms), ...])."""
+    n_ops = 2 * len(depths)
+    table = []
+    for per, wg in itertools.product(ops_per_pass, wg_per_cu):
+        if per > max(n_ops, 1) and per != min(ops_per_pass):
+            continue
+        best = float("inf")
+        for _ in range(repeats):
+            g = make_grid()
+            try:
+                g.reserve_batch(n_ops)
+                g.set_tuning(batch_ops_per_pass=per, integrate_wg_per_cu=wg)
+                torch.cuda.synchronize()
+                best = min(best, _timed(lambda: g.reintegrate_batch(depths, old_poses, new_poses)))
+            finally:
+                g.close()
+        table.append((per, wg, best))
+    per, wg, _ = min(table, key=lambda r: r[2])
+    return {"batch_ops_per_pass": per, "integrate_wg_per_cu": wg}, table
+
+
+def tune_integrate(grid, depths, poses, *, wg_per_cu=INTEGRATE_WG_PER_CU):
+    """Picks integrate_wg_per_cu for plain integrate calls on ``grid``.  Each
+    candidate integrates every frame and deintegrates it again, which leaves
+    the weights as they were (the sdf values may differ in the last bits).
+    Returns (best tuning dict, [(wg_per_cu, ms), ...])."""
+    table = []
+    for wg in wg_per_cu:
+        grid.set_tuning(integrate_wg_per_cu=wg)
+
+        def run():
+            for d, E in zip(depths, poses):
+                grid.integrate(None, d, E)
+            for d, E in zip(depths, poses):
+                grid.deIntegrate(None, d, E)
+
+        run()  # warm
+        table.append((wg, _timed(run)))
+    wg = min(table, key=lambda r: r[1])[0]
+    grid.set_tuning(integrate_wg_per_cu=wg)
+    return {"integrate_wg_per_cu": wg}, table
+
+
+def tune_raycast(grid, poses, *, xcd_map=(0, 1), superblocks=(0, 1)):
+    """Picks raycast_xcd_map and raycast_superblocks for ``grid.raycast``.
+    Returns (best tuning dict, [(xcd_map, superblocks, ms), ...])."""
+    table = []
+    for xm, sb in itertools.product(xcd_map, superblocks):
+        grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb)
+
+        def run():
+            for E in poses:
+                grid.raycast(E)
+
+        run()  # warm
+        table.append((xm, sb, _timed(run)))
+    xm, sb, _ = min(table, key=lambda r: r[2])
+    grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb)
+    return {"raycast_xcd_map": xm, "raycast_superblocks": sb}, table

@@ -598,7 +598,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, i
 // ---------------------------------------------------------------------------
 template <typename T, bool kColor>
 __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const OpDesc* ops, int par,
-                                                              Offsets off, uint32_t* op_mask)
+                                                              Offsets off, uint32_t* op_mask, int n_words)
 {
     const unsigned int count = g.ctrl->touched_count[par];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -613,8 +613,8 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
         uint32_t words[kMaskWords];
 #pragma unroll
         for (int k = 0; k < kMaskWords; ++k)
-            words[k] = op_mask[(size_t)slot * kMaskWords + k];
-        if (lane < kMaskWords)
+            words[k] = k < n_words ? op_mask[(size_t)slot * kMaskWords + k] : 0u;
+        if (lane < n_words)
             op_mask[(size_t)slot * kMaskWords + lane] = 0u;
         const uint32_t id = (uint32_t)g.slots[slot];
         if (id >= g.n_bricks)
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
         load_lane<T, kColor>(g, (int)id, lane, fresh, v);
         const LaneMid mid = lane_mid(g, off, c.x, c.y, c.z, lane);
         unsigned int dirty = fresh ? 0xFFu : 0u;
-        for (int k = 0; k < kMaskWords; ++k) {
+        for (int k = 0; k < n_words; ++k) {
             uint32_t bits = words[k];
             while (bits) {
                 const int b = __ffs(bits) - 1;
@@ -786,6 +786,8 @@ struct tsdf_grid {
     OpDesc* ops_dev;
     OpDesc* ops_host;  // pinned
     int batch_cap;     // point maps allocated in ptmap_batch
+    hipEvent_t staging_done;  // recorded after the last copy out of ops_host
+    bool staged;
     // raycast footprint buffers (allocated on first tsdf_raycast_footprint)
     uint32_t* fp_bits;
     unsigned long long* fp_keys;
@@ -919,15 +921,16 @@ void launch_integrate(const tsdf_grid* g, const OpDesc& op, int par, hipStream_t
 }
 
 template <typename T>
-void launch_integrate_batch(const tsdf_grid* g, int par, hipStream_t s, bool color)
+void launch_integrate_batch(const tsdf_grid* g, const OpDesc* ops, int n_ops, int par, hipStream_t s, bool color)
 {
     const int grid = persistent_grid(g);
+    const int words = (n_ops + 31) / 32;
     if (color)
-        hipLaunchKernelGGL((k_integrate_batch<T, true>), dim3(grid), dim3(kThreads), 0, s, view(g), g->ops_dev, par,
-                           g->off, g->op_mask);
+        hipLaunchKernelGGL((k_integrate_batch<T, true>), dim3(grid), dim3(kThreads), 0, s, view(g), ops, par,
+                           g->off, g->op_mask, words);
     else
-        hipLaunchKernelGGL((k_integrate_batch<T, false>), dim3(grid), dim3(kThreads), 0, s, view(g), g->ops_dev, par,
-                           g->off, g->op_mask);
+        hipLaunchKernelGGL((k_integrate_batch<T, false>), dim3(grid), dim3(kThreads), 0, s, view(g), ops, par,
+                           g->off, g->op_mask, words);
 }
 
 tsdf_status check_image_args(const tsdf_grid* g, const uint16_t* depth, int32_t pitch)
@@ -950,6 +953,7 @@ tsdf_status ensure_batch_buffers(tsdf_grid* g, int n_ops)
         HIPC(hipMemset(g->op_mask, 0, nb * kMaskWords * sizeof(uint32_t)));
         HIPC(hipMalloc(&g->ops_dev, kMaxOps * sizeof(OpDesc)));
         HIPC(hipHostMalloc(&g->ops_host, kMaxOps * sizeof(OpDesc), hipHostMallocDefault));
+        HIPC(hipEventCreateWithFlags(&g->staging_done, hipEventDisableTiming));
     }
     const int want = n_ops < kMaxOps ? n_ops : kMaxOps;
     if (g->batch_cap < want) {
@@ -1224,6 +1228,8 @@ void tsdf_destroy(tsdf_grid* g)
         (void)hipHostFree(g->ctrl_host);
     if (g->ops_host)
         (void)hipHostFree(g->ops_host);
+    if (g->staging_done)
+        (void)hipEventDestroy(g->staging_done);
     delete g;
 }
 
@@ -1326,36 +1332,47 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
     hipStream_t s = (hipStream_t)stream;
     const bool color = g->p.with_color != 0;
     const size_t frame = (size_t)g->p.height * g->p.width;
-    for (int base = 0; base < n_ops; base += kMaxOps) {
-        const int n = (n_ops - base) < kMaxOps ? (n_ops - base) : kMaxOps;
-        // the pinned staging buffer may still feed the previous chunk's copy
-        HIPC(hipStreamSynchronize(s));
-        for (int k = 0; k < n; ++k) {
+    // Ops go through the pinned staging buffer kMaxOps at a time; within that,
+    // passes of `per` ops each run one collection and one brick update.  A
+    // pass's point maps (per * H * W * 16 B) are rewritten by the next pass,
+    // so a small pass keeps them in the last-level cache.
+    const int per = g->tune.batch_ops_per_pass < g->batch_cap ? g->tune.batch_ops_per_pass : g->batch_cap;
+    for (int sbase = 0; sbase < n_ops; sbase += kMaxOps) {
+        const int sn = (n_ops - sbase) < kMaxOps ? (n_ops - sbase) : kMaxOps;
+        // the staging buffer may still feed the previous copy
+        if (g->staged)
+            HIPC(hipEventSynchronize(g->staging_done));
+        for (int k = 0; k < sn; ++k) {
             OpDesc& op = g->ops_host[k];
-            op.depth = depths[base + k];
-            op.rgb = (color && rgbs) ? rgbs[base + k] : nullptr;
-            op.ptmap = g->ptmap_batch + (size_t)k * frame;
+            op.depth = depths[sbase + k];
+            op.rgb = (color && rgbs) ? rgbs[sbase + k] : nullptr;
+            op.ptmap = g->ptmap_batch + (size_t)(k % per) * frame;
             op.pitch = pitch;
-            op.sign = signs[base + k];
-            make_pose(E + 16 * (size_t)(base + k), &op.pose);
+            op.sign = signs[sbase + k];
+            make_pose(E + 16 * (size_t)(sbase + k), &op.pose);
         }
-        HIPC(hipMemcpyAsync(g->ops_dev, g->ops_host, n * sizeof(OpDesc), hipMemcpyHostToDevice, s));
-        const uint32_t epoch = ++g->epoch;
-        const int par = (int)(epoch & 1u);
-        const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH, n);
-        {
-            ProfScope ps(g, s, TSDF_K_COLLECT_BATCH);
-            hipLaunchKernelGGL(k_collect_batch, cgrid, dim3(kThreads), 0, s, view(g), g->ops_dev, epoch, par,
-                               g->op_mask);
+        HIPC(hipMemcpyAsync(g->ops_dev, g->ops_host, sn * sizeof(OpDesc), hipMemcpyHostToDevice, s));
+        HIPC(hipEventRecord(g->staging_done, s));
+        g->staged = true;
+        for (int cb = 0; cb < sn; cb += per) {
+            const int n = (sn - cb) < per ? (sn - cb) : per;
+            const uint32_t epoch = ++g->epoch;
+            const int par = (int)(epoch & 1u);
+            const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH, n);
+            {
+                ProfScope ps(g, s, TSDF_K_COLLECT_BATCH);
+                hipLaunchKernelGGL(k_collect_batch, cgrid, dim3(kThreads), 0, s, view(g), g->ops_dev + cb, epoch,
+                                   par, g->op_mask);
+            }
+            {
+                ProfScope ps(g, s, TSDF_K_INTEGRATE_BATCH);
+                if (g->p.storage == TSDF_STORAGE_F16)
+                    launch_integrate_batch<__half>(g, g->ops_dev + cb, n, par, s, color);
+                else
+                    launch_integrate_batch<float>(g, g->ops_dev + cb, n, par, s, color);
+            }
+            HIPC(hipGetLastError());
         }
-        {
-            ProfScope ps(g, s, TSDF_K_INTEGRATE_BATCH);
-            if (g->p.storage == TSDF_STORAGE_F16)
-                launch_integrate_batch<__half>(g, par, s, color);
-            else
-                launch_integrate_batch<float>(g, par, s, color);
-        }
-        HIPC(hipGetLastError());
     }
     return TSDF_OK;
 }
@@ -1471,6 +1488,7 @@ void tsdf_default_tuning(tsdf_tuning* t)
     t->raycast_xcd_map = 1;
     t->raycast_superblocks = 0;
     t->integrate_wg_per_cu = 4;
+    t->batch_ops_per_pass = 16;
 }
 
 tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
@@ -1479,7 +1497,7 @@ tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
         return fail(TSDF_BAD_ARG, "NULL grid or tuning");
     if (t->raycast_xcd_map < 0 || t->raycast_xcd_map > 1 || t->raycast_superblocks < 0 ||
         t->raycast_superblocks > 1 || t->integrate_wg_per_cu < 1 ||
-        t->integrate_wg_per_cu > 32)
+        t->integrate_wg_per_cu > 32 || t->batch_ops_per_pass < 1 || t->batch_ops_per_pass > kMaxOps)
         return fail(TSDF_BAD_ARG, "workgroups per CU must lie in [1, 32]");
     g->tune = *t;
     return TSDF_OK;

@@ -242,7 +242,10 @@ class GPUVoxelGrid:
     def set_tuning(self, **kw):
         t = _lib.TsdfTuning()
         check(self._L.tsdf_get_tuning(self._h, C.byref(t)), "tsdf_get_tuning")
+        names = {n for n, _ in t._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise ValueError(f"unknown tuning field {k!r}; fields: {sorted(names)}")
             setattr(t, k, int(v))
         check(self._L.tsdf_set_tuning(self._h, C.byref(t)), "tsdf_set_tuning")
 

@@ -176,6 +176,36 @@ def test_batched_reintegration_equals_plain_calls(gpu):
     assert_raycast_equal(b, o, new[4])
 
 
+@pytest.mark.parametrize("per,wg", [(1, 4), (3, 2), (4, 8), (64, 3)])
+def test_batch_passes_equal_plain_calls(gpu, per, wg):
+    """The batch split into passes of `per` ops (any size, including ones
+    that do not divide the op count) and run at other occupancies still
+    equals the plain call sequence bit for bit; a second batch on the same
+    grid (reusing the staging buffer and point maps) too."""
+    cfg, intr, poses, depth = room(8, H=120, W=212)
+    rng = np.random.default_rng(40 + per)
+    new = [synth.pose_f32(synth.perturb(p.astype(np.float64), rng)) for p in poses]
+    newer = [synth.pose_f32(synth.perturb(p.astype(np.float64), rng)) for p in poses]
+    a, o = make_pair(cfg["voxel_length"], cfg["truncation"], 120, 212, intr)
+    b, _ = make_pair(cfg["voxel_length"], cfg["truncation"], 120, 212, intr)
+    b.set_tuning(batch_ops_per_pass=per, integrate_wg_per_cu=wg)
+    for k in range(8):
+        a.integrate(None, dev_u16(depth[k]), poses[k])
+        b.integrate(None, dev_u16(depth[k]), poses[k])
+    kf = [0, 2, 3, 5, 6, 7]
+    for src, dst in ((poses, new), (new, newer)):
+        for k in kf:
+            a.deIntegrate(None, dev_u16(depth[k]), src[k])
+            a.integrate(None, dev_u16(depth[k]), dst[k])
+        b.reintegrate_batch([dev_u16(depth[k]) for k in kf], [src[k] for k in kf], [dst[k] for k in kf])
+    ea, eb = a.export_bricks(), b.export_bricks()
+    oa = np.lexsort(ea["coords"].cpu().numpy().T[::-1])
+    ob = np.lexsort(eb["coords"].cpu().numpy().T[::-1])
+    assert np.array_equal(ea["coords"].cpu().numpy()[oa], eb["coords"].cpu().numpy()[ob])
+    for key in ("sdf", "w"):
+        assert_bits_equal(ea[key].cpu().numpy()[oa], eb[key].cpu().numpy()[ob], key)
+
+
 def test_empty_depth_is_noop(gpu):
     g, o = make_pair(0.005, 0.02, 120, 160, (130.0, 130.0, 79.5, 59.5))
     g.integrate(None, dev_u16(np.zeros((120, 160), np.uint16)), np.eye(4, dtype=np.float32), fence=True)
