@@ -121,6 +121,24 @@ class Runner:
             raise RuntimeError("a TSDF call failed: " + L.tsdf_last_error().decode())
 
 
+def autotune_plain(cfg, depth_dev, poses, frames):
+    """Launch shapes of the plain integrate and raycast, picked by the
+    autotuner on a scratch grid holding the first part of the sequence."""
+    from superscanner_amd import autotune
+    from superscanner_amd.voxel_grid import GPUVoxelGrid
+    g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], cfg["H"], cfg["W"],
+                     n_buckets=cfg["n_buckets"], n_bricks=1 << 21, max_weight=cfg["max_weight"])
+    g.setIntrinsic(*cfg["intr"])
+    n = min(frames, 80)
+    for k in range(n):
+        g.integrate(None, depth_dev[k], poses[k])
+    lo = max(0, n - 20)
+    best_i, _ = autotune.tune_integrate(g, [depth_dev[k] for k in range(lo, n)], [poses[k] for k in range(lo, n)])
+    best_r, _ = autotune.tune_raycast(g, [poses[k] for k in range(lo, n)])
+    g.close()
+    return {**best_i, **best_r}
+
+
 def footprint_pass(runner, n_samples):
     """Re-runs the sequence untimed and measures the raycast footprint
     (distinct voxels / keys) at n_samples evenly spaced poses."""
@@ -338,6 +356,9 @@ def main():
     grid = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
                         n_bricks=1 << 21, max_weight=cfg["max_weight"])
     grid.setIntrinsic(*cfg["intr"])
+    tuning = autotune_plain(cfg, depth_dev, poses, frames)
+    grid.set_tuning(**tuning)
+    print(f"[bench] autotuned launch shapes: {tuning}", file=sys.stderr, flush=True)
     run = Runner(grid, depth_dev, poses, frames)
 
     for _ in range(args.warmup):
@@ -423,6 +444,7 @@ def main():
         "kernels": kernels,
         "ms_per_step_profiled": round(ms_prof, 3),
         "raycast_footprint": {"u_vox": round(u_vox), "u_blk": round(u_blk), "samples": args.footprint_samples},
+        "tuning": tuning,
     }
     if not args.no_reintegration:
         out["batched_reintegration"] = reintegration(cfg, prims, poses64, args.keyframes, dev)

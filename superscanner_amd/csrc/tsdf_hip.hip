@@ -364,8 +364,8 @@ struct LaneVoxels {
     float c[kColor ? kVPE : 1][3];
 };
 
-// One integrate (kSign = +1) or deintegrate (kSign = -1) op applied to a
-// lane's eight voxels, in the reference's arithmetic order:
+// One integrate (kSign = +1) or deintegrate (kSign = -1; kSign = 0: op.sign)
+// op applied to a lane's eight voxels, in the reference's arithmetic order:
 //   c = transformPoint(vm, E_inv), (row, col) = getCoordinate(c)
 //   vpt = inputPcd(row, col) (from ptmap; x is NaN where z <= rmin, which
 //   makes sdf NaN and fails the |sdf| < trunc test exactly as the
@@ -375,7 +375,7 @@ struct LaneVoxels {
 //         semantics: the reference never caps, GPUVoxelGrid.h:14,62)
 //     -1: w != 1: s = (s w - sdf) / (w - 1), w = w - 1; else s = w = 0
 // Returns a bit per voxel that was updated.
-template <bool kColor, int kSign>
+template <bool kColor, int kSign, int kHalf = 4>
 __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc& op, float vmx, float vmy,
                                                  const float (&vmz)[kVPE], LaneVoxels<kColor>& v)
 {
@@ -385,24 +385,27 @@ __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc
     const float a0 = vmx * Ei[0] + vmy * Ei[1];
     const float a1 = vmx * Ei[4] + vmy * Ei[5];
     const float a2 = vmx * Ei[8] + vmy * Ei[9];
+    const float ez0 = op.pose.Ez[0], ez1 = op.pose.Ez[1], ez2 = op.pose.Ez[2];
+    const bool use_rgb = kColor && op.rgb != nullptr;
+    unsigned int upd = 0;
+    // two halves of kHalf voxels: the gathers of a half are issued together
+#pragma unroll
+    for (int h = 0; h < kVPE; h += kHalf) {
     int pix[kVPE];
 #pragma unroll
-    for (int z = 0; z < kVPE; ++z)
+    for (int z = h; z < h + kHalf; ++z)
         pix[z] = project_voxel(g, a0 + vmz[z] * Ei[2] + Ei[3], a1 + vmz[z] * Ei[6] + Ei[7],
                                a2 + vmz[z] * Ei[10] + Ei[11]);
     float pt[kVPE][3];
 #pragma unroll
-    for (int z = 0; z < kVPE; ++z) {
+    for (int z = h; z < h + kHalf; ++z) {
         const float* q = (const float*)(op.ptmap + (pix[z] >= 0 ? pix[z] : 0));
         pt[z][0] = q[0];
         pt[z][1] = q[1];
         pt[z][2] = q[2];
     }
-    const float ez0 = op.pose.Ez[0], ez1 = op.pose.Ez[1], ez2 = op.pose.Ez[2];
-    const bool use_rgb = kColor && op.rgb != nullptr;
-    unsigned int upd = 0;
 #pragma unroll
-    for (int z = 0; z < kVPE; ++z) {
+    for (int z = h; z < h + kHalf; ++z) {
         const float sdf = (pt[z][0] - vmx) * ez0 + (pt[z][1] - vmy) * ez1 + (pt[z][2] - vmz[z]) * ez2;
         if (pix[z] >= 0 && fabsf(sdf) < g.trunc) {
             upd |= 1u << z;
@@ -414,7 +417,21 @@ __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc
                 px[1] = (float)q[1];
                 px[2] = (float)q[2];
             }
-            if (kSign > 0) {
+            if (kSign == 0) {
+                // sign known only at run time (uniform per op): a - b is
+                // computed as a + (-b), which IEEE arithmetic makes identical
+                const bool pos = op.sign > 0;
+                const float sg = pos ? 1.0f : -1.0f;
+                const float d = w + sg;
+                const bool zero = !pos && w == 1.0f;
+                const bool cap = pos && g.wmax > 0.0f && w >= g.wmax;
+                const float q = (s * w + sg * sdf) / d;
+                v.s[z] = zero ? 0.0f : q;
+                if (use_rgb)
+                    for (int ch = 0; ch < 3; ++ch)
+                        v.c[z][ch] = zero ? 0.0f : (v.c[z][ch] * w + sg * px[ch]) / d;
+                v.w[z] = zero ? 0.0f : (cap ? g.wmax : d);
+            } else if (kSign > 0) {
                 const float d = w + 1.0f;
                 v.s[z] = (s * w + sdf) / d;
                 if (use_rgb)
@@ -431,6 +448,7 @@ __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc
                 v.w[z] = one ? 0.0f : d;
             }
         }
+    }
     }
     return upd;
 }
@@ -573,7 +591,9 @@ __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, i
         load_meta(g, t0, se, meta);
         __syncthreads();
         for (int e = wave; e < kMetaChunk; e += kThreads / 64) {
-            const int4 m = meta[e];
+            const int4 mv = meta[e];
+            const int4 m = make_int4(__builtin_amdgcn_readfirstlane(mv.x), __builtin_amdgcn_readfirstlane(mv.y),
+                                     __builtin_amdgcn_readfirstlane(mv.z), __builtin_amdgcn_readfirstlane(mv.w));
             if (m.x == -1)
                 continue;
             const bool fresh = m.x < 0;  // bit 31: allocated by this call
@@ -608,38 +628,40 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
     const int lane = threadIdx.x & 63;
     unsigned int nupd = 0;
     const WaveRange wr = wave_range(count);
+    // every value below that is the same for the whole wavefront is moved
+    // to scalar registers (readfirstlane)
     for (unsigned int t = wr.begin; t < wr.end; t += wr.stride) {
-        const uint32_t slot = g.touched[t];
-        uint32_t words[kMaskWords];
-#pragma unroll
-        for (int k = 0; k < kMaskWords; ++k)
-            words[k] = k < n_words ? op_mask[(size_t)slot * kMaskWords + k] : 0u;
-        if (lane < n_words)
-            op_mask[(size_t)slot * kMaskWords + lane] = 0u;
-        const uint32_t id = (uint32_t)g.slots[slot];
-        if (id >= g.n_bricks)
+        const uint32_t slot = __builtin_amdgcn_readfirstlane(g.touched[t]);
+        const uint32_t id = __builtin_amdgcn_readfirstlane((uint32_t)g.slots[slot]);
+        if (id >= g.n_bricks) {
+            if (lane < n_words)
+                op_mask[(size_t)slot * kMaskWords + lane] = 0u;
             continue;
+        }
         const int4 c = g.coords[id];
-        const bool fresh = (c.w & 1) != 0;
+        const int cw = __builtin_amdgcn_readfirstlane(c.w);
+        const bool fresh = (cw & 1) != 0;
         if (fresh && lane == 0)
-            g.coords[id].w = c.w & ~1;
+            g.coords[id].w = cw & ~1;
         LaneVoxels<kColor> v;
         load_lane<T, kColor>(g, (int)id, lane, fresh, v);
-        const LaneMid mid = lane_mid(g, off, c.x, c.y, c.z, lane);
+        const LaneMid mid = lane_mid(g, off, __builtin_amdgcn_readfirstlane(c.x), __builtin_amdgcn_readfirstlane(c.y),
+                                     __builtin_amdgcn_readfirstlane(c.z), lane);
         unsigned int dirty = fresh ? 0xFFu : 0u;
         for (int k = 0; k < n_words; ++k) {
-            uint32_t bits = words[k];
+            uint32_t bits = __builtin_amdgcn_readfirstlane(op_mask[(size_t)slot * kMaskWords + k]);
             while (bits) {
                 const int b = __ffs(bits) - 1;
                 bits &= bits - 1u;
                 const OpDesc& op = ops[k * 32 + b];
-                const unsigned int u = op.sign > 0 ? apply_op<kColor, 1>(g, op, mid.x, mid.y, mid.z, v)
-                                                   : apply_op<kColor, -1>(g, op, mid.x, mid.y, mid.z, v);
+                const unsigned int u = apply_op<kColor, 0>(g, op, mid.x, mid.y, mid.z, v);
                 dirty |= u;
                 nupd += __popc(u);
             }
         }
         store_lane<T, kColor>(g, (int)id, lane, dirty, v);
+        if (lane < n_words)  // every lane has read the mask words above
+            op_mask[(size_t)slot * kMaskWords + lane] = 0u;
     }
     nupd = wave_sum(nupd);
     if (lane == 0 && nupd)
