@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reintegration", action="store_true")
     ap.add_argument("--keyframes", type=int, default=64)
+    ap.add_argument("--no-diff-render", action="store_true")
     ap.add_argument("--footprint-samples", type=int, default=12)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run 2 steps of the workload plus a 256 MiB calibration copy and exit "
@@ -259,6 +260,64 @@ def reintegration(cfg, prims, poses64, keyframes, dev):
             "bit_identical": same}
 
 
+def diff_render_leg(dev, n_views=16, vh=240, vw=320, iters=20):
+    """Differentiable raycast over the plain_small TSDF (640x480 x 30 frames,
+    vl 5 mm): 16 views at 320x240, forward depth, adjoint (pose + voxel
+    gradients) and 20 Gauss-Newton tracking iterations per view."""
+    import torch
+    from superscanner_amd import diff_render as dr
+    from superscanner_amd import synth
+    from superscanner_amd.voxel_grid import GPUVoxelGrid
+    cfg = dict(synth.PLAIN_SMALL)
+    poses = synth.traj_plain_small(n=cfg["frames"])
+    depth = synth.frames(synth.scene_plain_small(), poses, cfg, seed=1)
+    g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], cfg["H"], cfg["W"],
+                     n_buckets=cfg["n_buckets"], n_bricks=1 << 18)
+    g.setIntrinsic(*cfg["intr"])
+    dd = torch.from_numpy(depth.view(np.int16)).to(dev)
+    for k in range(len(poses)):
+        g.integrate(None, dd[k], synth.pose_f32(poses[k]))
+    g.sync()
+    vi = synth.scaled_intr(cfg, vh, vw)
+    rng = np.random.default_rng(14)
+    true = np.stack([synth.perturb(poses[(2 * k) % len(poses)], rng, 0.01, 1.0) for k in range(n_views)])
+    init = np.stack([synth.perturb(t, rng, 0.01, 1.0) for t in true])
+    sdf = g.sdf_tensor()
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    dep = dr.render(g, true, vh, vw, vi, sdf=sdf)
+    gd = torch.randn_like(dep) * (dep > 0)
+    ms_fwd = timed(lambda: dr.render(g, true, vh, vw, vi, sdf=sdf))
+    ms_bwd = timed(lambda: dr.render_backward(g, true, vh, vw, vi, dep, gd, sdf=sdf))
+    obs = dr.render(g, true, vh, vw, vi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    T, hist = dr.track(g, obs, init, vi, iters=iters, max_residual=0.1)
+    t_track = time.perf_counter() - t0
+    err_t = [float(np.linalg.norm(a[:3, 3] - b[:3, 3])) for a, b in zip(T, true)]
+    err0 = [float(np.linalg.norm(a[:3, 3] - b[:3, 3])) for a, b in zip(init, true)]
+    g.close()
+    rays = n_views * vh * vw
+    return {"workload": f"{n_views} views {vw}x{vh} over the plain_small TSDF (640x480 x 30 frames, vl 5 mm)",
+            "hit_fraction": round(float((dep > 0).float().mean()), 3),
+            "ms_forward": round(ms_fwd, 3), "Mrays_per_s_forward": round(rays / ms_fwd / 1e3, 1),
+            "ms_backward": round(ms_bwd, 3),
+            "gauss_newton": {"iterations": iters, "ms_total": round(t_track * 1e3, 2),
+                             "ms_per_iteration_all_views": round(t_track * 1e3 / iters, 3),
+                             "median_translation_error_mm_init": round(1e3 * float(np.median(err0)), 3),
+                             "median_translation_error_mm_final": round(1e3 * float(np.median(err_t)), 4)}}
+
+
 CAL_BYTES = 256 << 20   # calibration copy: 256 MiB read + 256 MiB written
 KERNEL_NAMES = {"collect": "k_collect(", "integrate": "k_integrate<", "raycast": "k_raycast<"}
 
@@ -449,6 +508,9 @@ def main():
     if not args.no_reintegration:
         out["batched_reintegration"] = reintegration(cfg, prims, poses64, args.keyframes, dev)
         print("[bench] reintegration done", file=sys.stderr, flush=True)
+    if not args.no_diff_render:
+        out["diff_raycast"] = diff_render_leg(dev)
+        print("[bench] differentiable raycast done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, poses, depth, min(args.cpu_frames, frames))
         print("[bench] cpu baseline done", file=sys.stderr, flush=True)

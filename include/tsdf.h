@@ -113,6 +113,13 @@ typedef struct {
                                   /* batch, 1..256                              */
 } tsdf_tuning;
 
+/* Camera of a render view (the differentiable render has its own resolution
+ * and intrinsics; the reference convention ci = height - cy, cj = cx holds). */
+typedef struct {
+    int32_t height, width;
+    float fx, fy, cx, cy;
+} tsdf_view;
+
 /* Fills the defaults of the reference: d_max 10 m, rmin 0.2 m, no weight
  * cap, fp32 storage, no colour (GPUVoxelGrid.h:55-62). */
 void tsdf_default_params(tsdf_params *p);
@@ -173,6 +180,48 @@ tsdf_status tsdf_reserve_batch(tsdf_grid *g, int32_t n_ops);
  * point or NULL (extension).  ub_flags: device [H][W] uint8 or NULL. */
 tsdf_status tsdf_raycast(tsdf_grid *g, const float E_cam2world[16], float *points,
                          float *colors, float *depth, uint8_t *ub_flags, void *stream);
+
+/* ---- differentiable render (no reference counterpart) -----------------
+ * The reference raycast (GPUVoxelGrid.cpp:561-715) is not differentiable;
+ * these entry points render depth from the trilinear interpolation of the
+ * sdf (first zero crossing along the ray, refined by Newton's method) and
+ * give its exact derivative with respect to the pose and the voxel values
+ * (implicit function theorem at the root).  They replace the point-to-plane
+ * linearisation of SDF-to-pose tracking.  n_views <= 256 views share one
+ * camera; E: n_views*16 host floats (camera->world, row-major); all image
+ * buffers are DEVICE [n_views][h][w]; depth is camera-frame z in metres,
+ * 0 where no surface is found.  sdf: optional DEVICE [n_bricks][512] float
+ * values that replace the stored sdf (brick order of tsdf_export_bricks),
+ * NULL = the stored values; weights always come from the map. */
+
+/* normals: DEVICE [n][h][w][3] unit gradient of the field, or NULL. */
+tsdf_status tsdf_render(tsdf_grid *g, int32_t n_views, const float *E, const tsdf_view *cam,
+                        const float *sdf, float *depth, float *normals, void *stream);
+
+/* Adjoint of tsdf_render for a rendered `depth` and upstream gradient
+ * grad_depth = dL/dD: grad_pose DEVICE [n][6] (overwritten) = dL/dxi for the
+ * left perturbation E' = exp(xi^) E, xi = (translation, rotation);
+ * grad_sdf DEVICE [n_bricks][512] (accumulated) = dL/dV, or NULL. */
+tsdf_status tsdf_render_backward(tsdf_grid *g, int32_t n_views, const float *E, const tsdf_view *cam,
+                                 const float *sdf, const float *depth, const float *grad_depth,
+                                 float *grad_pose, float *grad_sdf, void *stream);
+
+/* Gauss-Newton normal equations of depth tracking against depth_obs
+ * (DEVICE [n][h][w] metres, 0 = none): per view, over pixels where the
+ * render hits, the observation is valid and |D - D_obs| < max_residual,
+ * J = dD/dxi and r = D - D_obs; normal_eq DEVICE [n][29] doubles
+ * (overwritten) = J^T J upper triangle row by row (21), J^T r (6), sum r^2,
+ * pixel count.  depth: DEVICE [n][h][w] rendered depth, or NULL. */
+tsdf_status tsdf_track_normal_eq(tsdf_grid *g, int32_t n_views, const float *E, const tsdf_view *cam,
+                                 const float *depth_obs, float max_residual, double *normal_eq,
+                                 float *depth, void *stream);
+
+/* Host, fp64: solves J^T J delta = -J^T r (Cholesky of the 6x6 system given
+ * as its upper triangle, 21 values as above) and composes
+ * T_out = exp(delta^) T_in (T row-major camera->world).  delta: 6 or NULL.
+ * TSDF_BAD_ARG when the system is not positive definite. */
+tsdf_status tsdf_se3_gauss_newton_step(const double JtJ[21], const double Jtr[6], const double T_in[16],
+                                       double T_out[16], double delta[6]);
 
 /* Defaults, current values and replacement of the launch shapes. */
 void tsdf_default_tuning(tsdf_tuning *t);

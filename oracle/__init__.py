@@ -84,6 +84,18 @@ def lib() -> C.CDLL:
     L.orc_export.restype = None
     L.orc_set_mutation.argtypes = [C.c_int]
     L.orc_set_mutation.restype = None
+    dp = P(C.c_double)
+    d = C.c_double
+    L.orc_field_create.argtypes = [C.c_int64, vp, vp, vp, C.c_float, C.c_float, C.c_float]
+    L.orc_field_create.restype = vp
+    L.orc_field_destroy.argtypes = [vp]
+    L.orc_field_destroy.restype = None
+    L.orc_field_set_sdf.argtypes = [vp, vp]
+    L.orc_field_set_sdf.restype = None
+    L.orc_render.argtypes = [vp, dp, C.c_int, C.c_int, d, d, d, d, vp, vp]
+    L.orc_render.restype = None
+    L.orc_render_grad.argtypes = [vp, dp, C.c_int, C.c_int, d, d, d, d, vp, vp, vp, vp, vp]
+    L.orc_render_grad.restype = None
     _lib = L
     return L
 
@@ -206,3 +218,57 @@ class OracleGrid:
         lib().orc_export(self._h, coords.ctypes.data, keys.ctypes.data, sdf.ctypes.data,
                          w.ctypes.data, rgb.ctypes.data if rgb is not None else None)
         return {"coords": coords, "keys": keys, "sdf": sdf, "w": w, "rgb": rgb}
+
+
+class OracleField:
+    """fp64 differentiable trilinear render of exported bricks (definition in
+    oracle/render_oracle.c; no reference counterpart)."""
+
+    def __init__(self, coords, sdf, w, voxel_length, min_depth=0.2, max_ray_length=10.0):
+        self.coords = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
+        self.n = self.coords.shape[0]
+        sdf = np.ascontiguousarray(sdf, dtype=np.float32).reshape(self.n, 512)
+        w = np.ascontiguousarray(w, dtype=np.float32).reshape(self.n, 512)
+        self._h = lib().orc_field_create(self.n, self.coords.ctypes.data, sdf.ctypes.data, w.ctypes.data,
+                                         voxel_length, min_depth, max_ray_length)
+
+    def close(self):
+        if self._h:
+            lib().orc_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_sdf(self, sdf):
+        a = np.ascontiguousarray(sdf, dtype=np.float32).reshape(self.n, 512)
+        lib().orc_field_set_sdf(self._h, a.ctypes.data)
+
+    @staticmethod
+    def _pose(E):
+        return np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(16))
+
+    def render(self, E, h, w, intr, normals=False):
+        """(depth [h, w] float64, normals [h, w, 3] or None); 0 = no surface."""
+        Ed = self._pose(E)
+        dep = np.zeros((h, w), np.float64)
+        nrm = np.zeros((h, w, 3), np.float64) if normals else None
+        lib().orc_render(self._h, Ed.ctypes.data_as(C.POINTER(C.c_double)), h, w, *map(float, intr),
+                         dep.ctypes.data, nrm.ctypes.data if nrm is not None else None)
+        return dep, nrm
+
+    def render_grad(self, E, h, w, intr, depth, grad_depth=None, want_sdf=False):
+        """(jac [h, w, 6] = dD/dxi, grad_pose [6], grad_sdf [n, 512] or None)."""
+        Ed = self._pose(E)
+        dep = np.ascontiguousarray(depth, dtype=np.float64).reshape(h, w)
+        gd = None if grad_depth is None else np.ascontiguousarray(grad_depth, dtype=np.float64).reshape(h, w)
+        jac = np.zeros((h, w, 6), np.float64)
+        gp = np.zeros(6, np.float64)
+        gs = np.zeros((self.n, 512), np.float64) if want_sdf else None
+        lib().orc_render_grad(self._h, Ed.ctypes.data_as(C.POINTER(C.c_double)), h, w, *map(float, intr),
+                              dep.ctypes.data, gd.ctypes.data if gd is not None else None, jac.ctypes.data,
+                              gp.ctypes.data, gs.ctypes.data if gs is not None else None)
+        return jac, gp, gs

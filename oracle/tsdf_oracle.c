@@ -39,6 +39,7 @@
  * catch a plausible error (0 = faithful oracle). */
 static int g_mut = 0;
 void orc_set_mutation(int m) { g_mut = m; }
+int orc_mutation(void) { return g_mut; }
 #define VPB 512  /* voxelPerBlock, GPUVoxelGrid.h:8 */
 
 typedef struct { float x, y, z; } f3;

@@ -62,8 +62,11 @@ void     orc_set_intrinsics(orc_grid *g, float fx, float fy, float cx, float cy)
 void     orc_set_threads(int n);
 /* Test-only: plants a deliberate mistake (0 = none, 1 sdf sign, 2 pose not
  * transposed, 3 floor block index, 4 textbook ci, 5 flipped interpolation,
- * 6 deintegrate keeps sdf at w == 1). */
+ * 6 deintegrate keeps sdf at w == 1, 7 render adjoint: rotation part with
+ * p x grad phi reversed, 8 render adjoint: voxel gradient without the 1/s
+ * factor). */
 void     orc_set_mutation(int m);
+int      orc_mutation(void);
 
 /* sign = +1 integrate, -1 deintegrate.  Returns 0, or 3 when the brick pool
  * overflowed (then the grid is no longer a valid reference state). */
@@ -83,6 +86,27 @@ int64_t  orc_n_bricks(const orc_grid *g);
  * order; voxel order x + 8y + 64z (GPUVoxelGrid.h:28-31). */
 void     orc_export(const orc_grid *g, int32_t *coords, int64_t *keys,
                     float *sdf, float *w, float *rgb);
+
+/* --- differentiable trilinear render (render_oracle.c), fp64 ---------
+ * No reference counterpart (the reference raycast is not differentiable);
+ * the definition of the field, march, root and adjoint is in
+ * render_oracle.c.  A field is built from exported bricks (coords n*3,
+ * sdf / w n*512, brick order of the export). */
+typedef struct orc_field orc_field;
+orc_field *orc_field_create(int64_t n, const int32_t *coords, const float *sdf, const float *w,
+                            float vl, float rmin, float dmax);
+void     orc_field_destroy(orc_field *f);
+void     orc_field_set_sdf(orc_field *f, const float *sdf);
+/* depth: h*w (0 = no surface), normal: h*w*3 or NULL.  E: camera->world,
+ * row-major, double. */
+void     orc_render(const orc_field *f, const double E[16], int h, int w, double fx, double fy,
+                    double cx, double cy, double *depth, double *normal);
+/* For the rendered depth: jac (h*w*6 or NULL) = dD/dxi per pixel (left
+ * perturbation xi = (v, w) of E), grad_pose (6 or NULL) = sum of
+ * grad_depth * jac, grad_sdf (n*512 or NULL) += grad_depth * dD/dV. */
+void     orc_render_grad(const orc_field *f, const double E[16], int h, int w, double fx, double fy,
+                         double cx, double cy, const double *depth, const double *grad_depth,
+                         double *jac, double *grad_pose, double *grad_sdf);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,7 @@ EXPORTS = (
     "tsdf_export_bricks", "tsdf_pose_prepare", "tsdf_status_string",
     "tsdf_last_error", "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
     "tsdf_default_tuning", "tsdf_get_tuning", "tsdf_set_tuning",
+    "tsdf_render", "tsdf_render_backward", "tsdf_track_normal_eq", "tsdf_se3_gauss_newton_step",
 )
 
 KERNEL_KINDS = ("reset", "collect", "integrate", "collect_batch", "integrate_batch", "raycast")
@@ -82,6 +83,11 @@ class TsdfKernelTimes(C.Structure):
                 for i, k in enumerate(KERNEL_KINDS)}
 
 
+class TsdfView(C.Structure):
+    _fields_ = [("height", C.c_int32), ("width", C.c_int32), ("fx", C.c_float), ("fy", C.c_float),
+                ("cx", C.c_float), ("cy", C.c_float)]
+
+
 class TsdfError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         self.status = status
@@ -115,6 +121,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_integrate_batch.argtypes = [vp, C.c_int32, P(vp), C.c_int32, P(vp), fp, P(C.c_int32), vp]
     L.tsdf_reintegrate_batch.argtypes = [vp, C.c_int32, P(vp), C.c_int32, P(vp), fp, fp, vp]
     L.tsdf_reserve_batch.argtypes = [vp, C.c_int32]
+    dp = P(C.c_double)
+    L.tsdf_render.argtypes = [vp, C.c_int32, fp, P(TsdfView), vp, vp, vp, vp]
+    L.tsdf_render_backward.argtypes = [vp, C.c_int32, fp, P(TsdfView), vp, vp, vp, vp, vp, vp]
+    L.tsdf_track_normal_eq.argtypes = [vp, C.c_int32, fp, P(TsdfView), vp, C.c_float, vp, vp, vp]
+    L.tsdf_se3_gauss_newton_step.argtypes = [dp, dp, dp, dp, dp]
     L.tsdf_raycast.argtypes = [vp, fp, vp, vp, vp, vp, vp]
     L.tsdf_sync.argtypes = [vp, vp]
     L.tsdf_stats.argtypes = [vp, P(TsdfStats), vp]

@@ -69,10 +69,6 @@ constexpr int kMaxAxisBlocks = 64;
 // small device helpers
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ float ld(const float* p) { return *p; }
-__device__ __forceinline__ float ld(const __half* p) { return __half2float(*p); }
-__device__ __forceinline__ void st(float* p, float v) { *p = v; }
-__device__ __forceinline__ void st(__half* p, float v) { *p = __float2half_rn(v); }
 
 __device__ __forceinline__ unsigned long long pack_coord(int x, int y, int z)
 {
@@ -349,7 +345,9 @@ __device__ __forceinline__ int project_voxel(const GridView& g, float cx, float 
     const float r = __builtin_amdgcn_rcpf(cz);
     const float ny = cy * g.cam.fy, nx = cx * g.cam.fx;
     int i, j;
-    const bool ok = rcp_certified(r, cz) & pix_axis_fast(ny, r, g.cam.ci, i) & pix_axis_fast(nx, r, g.cam.cj, j);
+    // both axes are evaluated (no short circuit), then one branch
+    const bool ok = ((int)rcp_certified(r, cz) & (int)pix_axis_fast(ny, r, g.cam.ci, i) &
+                     (int)pix_axis_fast(nx, r, g.cam.cj, j)) != 0;
     if (!ok) {
         i = pix_axis_ref(ny, cz, g.cam.ci);
         j = pix_axis_ref(nx, cz, g.cam.cj);
@@ -824,6 +822,11 @@ struct tsdf_grid {
     double prof_ms[TSDF_N_KERNEL_KINDS];
     uint64_t prof_n[TSDF_N_KERNEL_KINDS];
     tsdf_tuning tune;
+    // render views (differentiable render / tracking)
+    ViewPose* view_dev;
+    ViewPose* view_host;  // pinned
+    hipEvent_t render_done;
+    bool render_staged;
 };
 
 namespace {
@@ -1201,10 +1204,15 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
     alloc((void**)&g->sb_bits, kSbWords * sizeof(uint32_t));
     alloc((void**)&g->ptmap, (size_t)p->height * p->width * sizeof(float4));
     alloc((void**)&g->ctrl, sizeof(Ctrl));
+    alloc((void**)&g->view_dev, 256 * sizeof(ViewPose));
     if (st == TSDF_OK) {
         hipError_t e = hipHostMalloc((void**)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&g->view_host, 256 * sizeof(ViewPose), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&g->render_done, hipEventDisableTiming);
         if (e != hipSuccess)
-            st = fail(TSDF_HIP_ERROR, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+            st = fail(TSDF_HIP_ERROR, std::string("hipHostMalloc / hipEventCreate: ") + hipGetErrorString(e));
     }
     if (st == TSDF_OK)
         st = tsdf_reset(g, nullptr);
@@ -1248,6 +1256,11 @@ void tsdf_destroy(tsdf_grid* g)
     free(g->prof_kind);
     if (g->ctrl_host)
         (void)hipHostFree(g->ctrl_host);
+    if (g->view_host)
+        (void)hipHostFree(g->view_host);
+    if (g->render_done)
+        (void)hipEventDestroy(g->render_done);
+    (void)hipFree(g->view_dev);
     if (g->ops_host)
         (void)hipHostFree(g->ops_host);
     if (g->staging_done)
@@ -1623,3 +1636,5 @@ tsdf_status tsdf_export_bricks(tsdf_grid* g, int32_t* coords, float* sdf, float*
 }
 
 }  // extern "C"
+
+#include "tsdf_render.hpp"

@@ -181,4 +181,11 @@ struct Pose {
     float pad;
 };
 
+// Pose of a render view: camera -> world rotation (row-major) and the camera
+// centre in world coordinates.
+struct ViewPose {
+    float R[9];
+    float t[3];
+};
+
 }  // namespace tsdf

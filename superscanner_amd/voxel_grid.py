@@ -261,6 +261,12 @@ class GPUVoxelGrid:
         """voxelMap2.size()"""
         return self.stats()["n_bricks"]
 
+    def sdf_tensor(self) -> torch.Tensor:
+        """The stored sdf values as a device float32 tensor [n_bricks, 512]
+        (row = brick id, voxel x + 8y + 64z): the voxel parameters of the
+        differentiable render (superscanner_amd.diff_render)."""
+        return self.export_bricks()["sdf"]
+
     def export_bricks(self, stream=None) -> dict:
         n = self.size()
         dev = self.device
