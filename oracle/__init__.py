@@ -96,6 +96,16 @@ def lib() -> C.CDLL:
     L.orc_render.restype = None
     L.orc_render_grad.argtypes = [vp, dp, C.c_int, C.c_int, d, d, d, d, vp, vp, vp, vp, vp]
     L.orc_render_grad.restype = None
+    L.orc64_create.argtypes = [d, d, C.c_int, C.c_int, d, d, d, d, dp, dp, d, d]
+    L.orc64_create.restype = vp
+    L.orc64_destroy.argtypes = [vp]
+    L.orc64_destroy.restype = None
+    L.orc64_integrate.argtypes = [vp, vp, C.c_int, dp, C.c_int]
+    L.orc64_integrate.restype = None
+    L.orc64_n_bricks.argtypes = [vp]
+    L.orc64_n_bricks.restype = C.c_int64
+    L.orc64_export.argtypes = [vp, vp, vp, vp]
+    L.orc64_export.restype = None
     _lib = L
     return L
 
@@ -272,3 +282,45 @@ class OracleField:
                               dep.ctypes.data, gd.ctypes.data if gd is not None else None, jac.ctypes.data,
                               gp.ctypes.data, gs.ctypes.data if gs is not None else None)
         return jac, gp, gs
+
+
+class Oracle64Grid:
+    """The integrate in double precision on the blocks meeting a crop box
+    (oracle/fp64_oracle.c): the fp64 reference for the error of the fp32 /
+    fp16 GPU grids."""
+
+    def __init__(self, voxel_length, truncation, height, width, intr, crop_min, crop_max,
+                 max_weight=0.0, min_depth=0.2):
+        self.H, self.W = int(height), int(width)
+        lo = np.ascontiguousarray(crop_min, np.float64)
+        hi = np.ascontiguousarray(crop_max, np.float64)
+        dp = C.POINTER(C.c_double)
+        self._h = lib().orc64_create(float(voxel_length), float(truncation), self.H, self.W,
+                                     *map(float, intr), lo.ctypes.data_as(dp), hi.ctypes.data_as(dp),
+                                     float(max_weight), float(min_depth))
+
+    def close(self):
+        if self._h:
+            lib().orc64_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def integrate(self, depth, E, sign=1):
+        d = np.ascontiguousarray(depth, dtype=np.uint16)
+        assert d.shape == (self.H, self.W)
+        Ed = np.ascontiguousarray(np.asarray(E, np.float64).reshape(16))
+        lib().orc64_integrate(self._h, d.ctypes.data, self.W, Ed.ctypes.data_as(C.POINTER(C.c_double)), int(sign))
+
+    def export(self):
+        n = int(lib().orc64_n_bricks(self._h))
+        coords = np.zeros((n, 3), np.int32)
+        sdf = np.zeros((n, 512), np.float64)
+        w = np.zeros((n, 512), np.float64)
+        if n:
+            lib().orc64_export(self._h, coords.ctypes.data, sdf.ctypes.data, w.ctypes.data)
+        return {"coords": coords, "sdf": sdf, "w": w}

@@ -108,6 +108,20 @@ void     orc_render_grad(const orc_field *f, const double E[16], int h, int w, d
                          double cx, double cy, const double *depth, const double *grad_depth,
                          double *jac, double *grad_pose, double *grad_sdf);
 
+/* --- fp64 integrate on a crop (fp64_oracle.c) ------------------------
+ * The integrate of orc_integrate evaluated in double, keeping only blocks
+ * whose box meets [crop_min, crop_max]; measures the error of the fp32 /
+ * fp16 GPU grids.  E: camera->world, row-major, double. */
+typedef struct orc64_grid orc64_grid;
+orc64_grid *orc64_create(double vl, double trunc, int H, int W, double fx, double fy, double cx, double cy,
+                         const double crop_min[3], const double crop_max[3], double max_weight,
+                         double min_depth);
+void     orc64_destroy(orc64_grid *g);
+void     orc64_integrate(orc64_grid *g, const uint16_t *depth, int pitch, const double E[16], int sign);
+int64_t  orc64_n_bricks(const orc64_grid *g);
+/* coords n*3 int32, sdf / w n*512 double (voxel x + 8y + 64z). */
+void     orc64_export(const orc64_grid *g, int32_t *coords, double *sdf, double *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,8 @@ configurations of this path:
   reintegrate   room_large scene, 64 keyframes, poses perturbed by
                 N(0, 1 cm / 0.5 deg), per-keyframe valid fraction U(0.3, 1)
   diff_raycast  16 views at 320x240 over the plain_small TSDF
-  fp16_large    1280x720 x 2000 frames of a 12 m x 8 m multi-room, vl 2 mm,
+  fp16_large    1280x720 x 2000 frames of a 12 m x 8 m multi-room
+                (scene_multiroom / traj_multiroom), vl 2 mm, trunc 4 voxels,
                 2^24 buckets, fp16 storage
 
 Depth is rendered by libtsdf_synth.so (double-precision ray casting with the
@@ -176,6 +177,82 @@ def traj_random_walk(n=600, seed=2, step_m=0.005, step_deg=0.5, start=(0.0, 0.0,
         T[:3, 3] += step_m * heading
         ax = rng.normal(size=3) * np.array([0.3, 0.3, 1.0])
         T[:3, :3] = axis_angle(ax, math.radians(step_deg)) @ T[:3, :3]
+    return poses
+
+
+# multi-room floor plan: 3 x 2 rooms of 4 m x 4 m in a 12 m x 8 m x 2.6 m
+# shell, 0.1 m interior walls with 1 m doors
+MULTIROOM = dict(size=(12.0, 8.0, 2.6), rooms=(3, 2), room=4.0, wall=0.1, door=1.0)
+
+
+def scene_multiroom(seed=5):
+    """Seeded multi-room scene: outer shell, interior walls with doors, and
+    in every room 3 boxes and 1 sphere placed in the room's corners (the
+    trajectory runs along the rooms' centre lines)."""
+    rng = np.random.default_rng(seed)
+    sx, sy, sz = MULTIROOM["size"]
+    t, door, rs = MULTIROOM["wall"], MULTIROOM["door"], MULTIROOM["room"]
+    prims = [(ROOM, (-sx / 2, -sy / 2, 0.0, sx / 2, sy / 2, sz))]
+    # walls x = -2, 2 (along y), one door per room they separate
+    for wx in (-2.0, 2.0):
+        y0 = -sy / 2
+        for dy in (-2.0, 2.0):
+            prims.append((BOX, (wx - t / 2, y0, 0.0, wx + t / 2, dy - door / 2, sz)))
+            y0 = dy + door / 2
+        prims.append((BOX, (wx - t / 2, y0, 0.0, wx + t / 2, sy / 2, sz)))
+    # wall y = 0 (along x), one door per room
+    x0 = -sx / 2
+    for dx in (-4.0, 0.0, 4.0):
+        prims.append((BOX, (x0, -t / 2, 0.0, dx - door / 2, t / 2, sz)))
+        x0 = dx + door / 2
+    prims.append((BOX, (x0, -t / 2, 0.0, sx / 2, t / 2, sz)))
+    for cx in (-4.0, 0.0, 4.0):
+        for cy in (-2.0, 2.0):
+            corners = [(1, 1), (1, -1), (-1, 1), (-1, -1)]
+            rng.shuffle(corners)
+            for k, (qx, qy) in enumerate(corners):
+                px = cx + qx * rng.uniform(1.1, 1.4)
+                py = cy + qy * rng.uniform(1.1, 1.4)
+                if k < 3:
+                    w, d, h = rng.uniform(0.25, 0.5, 2).tolist() + [rng.uniform(0.3, 1.2)]
+                    prims.append((BOX, (px - w / 2, py - d / 2, 0.0, px + w / 2, py + d / 2, h)))
+                else:
+                    r = rng.uniform(0.15, 0.3)
+                    prims.append((SPHERE, (px, py, rng.uniform(r, 1.5), r)))
+    del rs
+    return prims
+
+
+def traj_multiroom(n=2000, seed=6, height=1.4):
+    """Loop through all six rooms via their doors (room centres and door
+    centres as waypoints, about 24 m), looking along the direction of travel
+    with a slow yaw sweep of +-35 deg and a pitch of -12 deg +- 5 deg."""
+    rng = np.random.default_rng(seed)
+    way = [(-4, -2), (-2, -2), (0, -2), (2, -2), (4, -2), (4, 0), (4, 2), (2, 2), (0, 2), (-2, 2),
+           (-4, 2), (-4, 0), (-4, -2)]
+    way = np.array(way, np.float64)
+    seg = np.linalg.norm(np.diff(way, axis=0), axis=1)
+    cum = np.concatenate([[0.0], np.cumsum(seg)])
+    L = cum[-1]
+    ph = rng.uniform(0, 2 * math.pi, 2)
+    poses = []
+    for k in range(n):
+        s = L * k / n
+        i = min(int(np.searchsorted(cum, s, side="right")) - 1, len(seg) - 1)
+        u = (s - cum[i]) / seg[i]
+        pos = way[i] + u * (way[i + 1] - way[i])
+        # heading: tangent blended toward the next segment near its end
+        tan = (way[i + 1] - way[i]) / seg[i]
+        if i + 1 < len(seg) and u > 0.7:
+            nxt = (way[i + 2] - way[i + 1]) / seg[i + 1]
+            a = (u - 0.7) / 0.3
+            tan = (1 - a) * tan + a * nxt
+            tan /= np.linalg.norm(tan)
+        yaw = math.atan2(tan[1], tan[0]) + math.radians(35.0) * math.sin(2 * math.pi * k / 400.0 + ph[0])
+        pitch = math.radians(-12.0 + 5.0 * math.sin(2 * math.pi * k / 300.0 + ph[1]))
+        eye = np.array([pos[0], pos[1], height])
+        fwd = np.array([math.cos(yaw) * math.cos(pitch), math.sin(yaw) * math.cos(pitch), math.sin(pitch)])
+        poses.append(look_at(eye, eye + fwd))
     return poses
 
 
