@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-reintegration", action="store_true")
     ap.add_argument("--keyframes", type=int, default=64)
     ap.add_argument("--no-diff-render", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--fp16-frames", type=int, default=2000)
     ap.add_argument("--footprint-samples", type=int, default=12)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run 2 steps of the workload plus a 256 MiB calibration copy and exit "
@@ -318,6 +320,76 @@ def diff_render_leg(dev, n_views=16, vh=240, vw=320, iters=20):
                              "median_translation_error_mm_final": round(1e3 * float(np.median(err_t)), 4)}}
 
 
+def fp16_scale_leg(dev, frames):
+    """fp16 storage at scale: the multi-room scene (12 m x 8 m), 1280x720
+    uint16 depth, vl 2 mm, 2^24 buckets; integrate each frame and raycast
+    each pose with fp16 (sdf, w), then the same sequence with fp32 storage.
+    Reports the fp16 throughput and the fp16-vs-fp32 difference of the two
+    maps (storage error at scale; the error against an fp64 oracle is
+    measured on a crop by tests/test_gpu_fp16_scale.py)."""
+    import torch
+    from superscanner_amd import synth
+    from superscanner_amd.voxel_grid import GPUVoxelGrid
+    cfg = dict(synth.FP16_LARGE)
+    H, W = cfg["H"], cfg["W"]
+    poses64 = synth.traj_multiroom(frames)
+    t0 = time.time()
+    depth = synth.frames(synth.scene_multiroom(), poses64, cfg, seed=12)
+    gen_s = time.time() - t0
+    poses = np.stack([synth.pose_f32(T) for T in poses64])
+    dd = torch.from_numpy(depth.view(np.int16)).to(dev)
+    del depth
+    out = {"workload": f"multi-room 12x8 m, {frames} frames {W}x{H}, vl 2 mm, trunc 8 mm, 2^24 buckets; "
+                       "integrate each frame, raycast each pose",
+           "frames": frames, "generation_s": round(gen_s, 1)}
+    maps = {}
+    for st in ("f16", "f32"):
+        g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
+                         n_bricks=1 << 23, storage=st)
+        g.setIntrinsic(*cfg["intr"])
+        run = Runner(g, dd, poses, frames)
+        run.ops = [o for k in range(frames) for o in (("i", k), ("r", k))]
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        run.step()
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b)
+        stats = g.stats()
+        g.sync()
+        bpv = 2 if st == "f16" else 4
+        out[st] = {"ms": round(ms, 1), "ms_per_frame": round(ms / frames, 3),
+                   "Mpx_per_s": round(frames * H * W / ms / 1e3, 1), "bricks": stats["n_bricks"],
+                   "payload_GB": round(stats["n_bricks"] * 512 * 2 * bpv / 1e9, 2)}
+        e = g.export_bricks()
+        c = e["coords"].long()
+        key = ((c[:, 0] + (1 << 20)) << 42) | ((c[:, 1] + (1 << 20)) << 21) | (c[:, 2] + (1 << 20))
+        order = torch.argsort(key)
+        maps[st] = {"key": key[order], "sdf": e["sdf"][order], "w": e["w"][order]}
+        # depth of the last pose, for the map comparison
+        g.raycast(poses[frames - 1])
+        maps[st]["depth"] = g.rayDepth.clone()
+        del e, c, key, order
+        g.close()
+        print(f"[bench] fp16 leg: {st} storage done", file=sys.stderr, flush=True)
+    a16, a32 = maps["f16"], maps["f32"]
+    same_keys = bool(torch.equal(a16["key"], a32["key"]))
+    out["same_brick_set"] = same_keys
+    if same_keys:
+        obs = a16["w"] > 0
+        d = (a16["sdf"] - a32["sdf"]).abs()[obs]
+        out["f16_vs_f32"] = {"weights_identical": bool(torch.equal(a16["w"], a32["w"])),
+                             "observed_voxels": int(obs.sum()),
+                             "max_abs_sdf_diff_m": float(d.max()),
+                             "rms_sdf_diff_m": float(d.pow(2).mean().sqrt())}
+    both = (a16["depth"] > 0) & (a32["depth"] > 0)
+    dz = (a16["depth"] - a32["depth"])[both]
+    out.setdefault("f16_vs_f32", {})["raycast_depth_rmse_m"] = float(dz.pow(2).mean().sqrt()) if dz.numel() else None
+    out["speedup_f16_vs_f32"] = round(out["f32"]["ms"] / out["f16"]["ms"], 3)
+    return out
+
+
 CAL_BYTES = 256 << 20   # calibration copy: 256 MiB read + 256 MiB written
 KERNEL_NAMES = {"collect": "k_collect(", "integrate": "k_integrate<", "raycast": "k_raycast<"}
 
@@ -511,6 +583,9 @@ def main():
     if not args.no_diff_render:
         out["diff_raycast"] = diff_render_leg(dev)
         print("[bench] differentiable raycast done", file=sys.stderr, flush=True)
+    if not args.no_fp16:
+        out["fp16_large"] = fp16_scale_leg(dev, args.fp16_frames)
+        print("[bench] fp16 at scale done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, poses, depth, min(args.cpu_frames, frames))
         print("[bench] cpu baseline done", file=sys.stderr, flush=True)
