@@ -171,6 +171,23 @@ __device__ __forceinline__ bool claim(const GridView& g, int x, int y, int z, ui
     return atomicExch(&g.slot_epoch[s], epoch) != epoch;
 }
 
+// Bits [a, b) of one 64-bit word, 0 <= a < b <= 64.
+__device__ __forceinline__ unsigned long long bit_range(int a, int b)
+{
+    const unsigned long long hi = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
+    return hi & ~((1ull << a) - 1ull);
+}
+
+// Sets bits [base, base + n) of the 128-bit value (lo, hi), base + n <= 128.
+__device__ __forceinline__ void set_bits128(unsigned long long& lo, unsigned long long& hi, int base, int n)
+{
+    const int e = base + n;
+    if (base < 64)
+        lo |= bit_range(base, e < 64 ? e : 64);
+    if (e > 64)
+        hi |= bit_range(base > 64 ? base - 64 : 0, e - 64);
+}
+
 __device__ __forceinline__ bool lds_insert(unsigned long long* set, unsigned long long v)
 {
     uint32_t h = mix32((uint32_t)v ^ ((uint32_t)(v >> 32) * 0x9E3779B1u)) & (kLdsSet - 1);
@@ -239,45 +256,107 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
         s_valid = 0;
     __syncthreads();
 
-    const int tx = threadIdx.x & (kTileW - 1);
-    const int ty = threadIdx.x / kTileW;
-    const int j = blockIdx.x * kTileW + tx;
+    // Each wavefront takes 8 x 8 pixel patches of the 64 x 16 tile (lane =
+    // pixel of the patch).  The blocks of a patch are the union of its
+    // pixels' +-trunc boxes; when their common bounding box has at most 128
+    // cells, that union is built exactly as a 128-bit mask over the box (each
+    // lane ORs in its own box, then the wave ORs the masks) and every lane
+    // inserts at most two distinct blocks into the tile's set.  Larger unions
+    // fall back to per-pixel enumeration.
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
     unsigned int nvalid = 0;
-    for (int r = 0; r < kTileH / (kThreads / kTileW); ++r) {
-        const int i = blockIdx.y * kTileH + ty + r * (kThreads / kTileW);
-        if (i >= g.H || j >= g.W)
-            continue;
-        const float d = (float)op.depth[(size_t)i * op.pitch + j] / 1000.0f;
-        const f3 p = get_point(g.cam, i, j, d);
-        // pixels without a measurement (z <= rmin) carry x = NaN: any sdf
-        // computed from them is NaN and fails the |sdf| < trunc test
-        float4 o = make_float4(__builtin_nanf(""), 0.f, 0.f, 0.f);
-        if (p.z > g.rmin) {
-            ++nvalid;
-            const f3 pt = xform(p, op.pose.E);
-            o = make_float4(pt.x, pt.y, pt.z, 1.0f);
-            const int x0 = div_index(pt.x - g.trunc, g.vbl, g.rvbl), x1 = div_index(pt.x + g.trunc, g.vbl, g.rvbl);
-            const int y0 = div_index(pt.y - g.trunc, g.vbl, g.rvbl), y1 = div_index(pt.y + g.trunc, g.vbl, g.rvbl);
-            const int z0 = div_index(pt.z - g.trunc, g.vbl, g.rvbl), z1 = div_index(pt.z + g.trunc, g.vbl, g.rvbl);
-            const bool sane = (long long)x1 - x0 < kMaxAxisBlocks && (long long)y1 - y0 < kMaxAxisBlocks &&
-                              (long long)z1 - z0 < kMaxAxisBlocks;
-            if (sane) {
-                for (int x = x0; x <= x1; ++x)
-                    for (int y = y0; y <= y1; ++y)
-                        for (int z = z0; z <= z1; ++z) {
-                            const bool ok = packable(x) && packable(y) && packable(z) &&
-                                            lds_insert(lset, pack_coord(x, y, z));
-                            if (!ok) {  // set full or far block: claim directly
+    for (int pt_idx = wv; pt_idx < (kTileW / 8) * (kTileH / 8); pt_idx += kThreads / 64) {
+        const int i = blockIdx.y * kTileH + (pt_idx / (kTileW / 8)) * 8 + (lane >> 3);
+        const int j = blockIdx.x * kTileW + (pt_idx % (kTileW / 8)) * 8 + (lane & 7);
+        const bool in = i < g.H && j < g.W;
+        int x0 = 0x7FFFFFFF, y0 = 0x7FFFFFFF, z0 = 0x7FFFFFFF;
+        int x1 = (int)0x80000000u, y1 = (int)0x80000000u, z1 = (int)0x80000000u;
+        bool box = false;  // this pixel contributes a box
+        if (in) {
+            const float d = (float)op.depth[(size_t)i * op.pitch + j] / 1000.0f;
+            const f3 p = get_point(g.cam, i, j, d);
+            // pixels without a measurement (z <= rmin) carry x = NaN: any sdf
+            // computed from them is NaN and fails the |sdf| < trunc test
+            float4 o = make_float4(__builtin_nanf(""), 0.f, 0.f, 0.f);
+            if (p.z > g.rmin) {
+                ++nvalid;
+                const f3 pw = xform(p, op.pose.E);
+                o = make_float4(pw.x, pw.y, pw.z, 1.0f);
+                const int bx0 = div_index(pw.x - g.trunc, g.vbl, g.rvbl), bx1 = div_index(pw.x + g.trunc, g.vbl, g.rvbl);
+                const int by0 = div_index(pw.y - g.trunc, g.vbl, g.rvbl), by1 = div_index(pw.y + g.trunc, g.vbl, g.rvbl);
+                const int bz0 = div_index(pw.z - g.trunc, g.vbl, g.rvbl), bz1 = div_index(pw.z + g.trunc, g.vbl, g.rvbl);
+                const bool sane = (long long)bx1 - bx0 < kMaxAxisBlocks && (long long)by1 - by0 < kMaxAxisBlocks &&
+                                  (long long)bz1 - bz0 < kMaxAxisBlocks;
+                const bool pk = packable(bx0) && packable(bx1) && packable(by0) && packable(by1) && packable(bz0) &&
+                                packable(bz1);
+                if (sane && pk) {
+                    x0 = bx0; x1 = bx1; y0 = by0; y1 = by1; z0 = bz0; z1 = bz1;
+                    box = true;
+                } else if (!sane) {
+                    atomicAdd(&g.ctrl->overflow, 1u);
+                } else {
+                    // a block index beyond the packable range: claim directly
+                    for (int x = bx0; x <= bx1; ++x)
+                        for (int y = by0; y <= by1; ++y)
+                            for (int z = bz0; z <= bz1; ++z) {
                                 uint32_t slot;
                                 if (claim<kBatch>(g, x, y, z, epoch, op_mask, op_index, &slot))
                                     g.touched[atomicAdd(&g.ctrl->touched_count[par], 1u)] = slot;
                             }
-                        }
-            } else {
-                atomicAdd(&g.ctrl->overflow, 1u);
+                }
             }
+            op.ptmap[(size_t)i * g.W + j] = o;
         }
-        op.ptmap[(size_t)i * g.W + j] = o;
+        if (__ballot(box) == 0ull)
+            continue;
+        // bounding box of the patch's boxes (wave reduction)
+        int ux0 = x0, uy0 = y0, uz0 = z0, ux1 = x1, uy1 = y1, uz1 = z1;
+        for (int m = 32; m > 0; m >>= 1) {
+            ux0 = min(ux0, __shfl_xor(ux0, m));
+            uy0 = min(uy0, __shfl_xor(uy0, m));
+            uz0 = min(uz0, __shfl_xor(uz0, m));
+            ux1 = max(ux1, __shfl_xor(ux1, m));
+            uy1 = max(uy1, __shfl_xor(uy1, m));
+            uz1 = max(uz1, __shfl_xor(uz1, m));
+        }
+        const long long ex = (long long)ux1 - ux0 + 1, ey = (long long)uy1 - uy0 + 1, ez = (long long)uz1 - uz0 + 1;
+        if (ex * ey * ez <= 128) {
+            const int nx = (int)ex, nxy = (int)(ex * ey);
+            unsigned long long mlo = 0ull, mhi = 0ull;
+            if (box)
+                for (int z = z0; z <= z1; ++z)
+                    for (int y = y0; y <= y1; ++y)
+                        set_bits128(mlo, mhi, (x0 - ux0) + nx * (y - uy0) + nxy * (z - uz0), x1 - x0 + 1);
+            for (int m = 32; m > 0; m >>= 1) {
+                mlo |= __shfl_xor(mlo, m);
+                mhi |= __shfl_xor(mhi, m);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c = lane + 64 * h;
+                const unsigned long long word = h ? mhi : mlo;
+                if (!((word >> lane) & 1ull))
+                    continue;
+                const int cz = c / nxy, rem = c - cz * nxy;
+                const int cy = rem / nx, cx = rem - cy * nx;
+                const int x = ux0 + cx, y = uy0 + cy, z = uz0 + cz;
+                if (!lds_insert(lset, pack_coord(x, y, z))) {  // set full: claim directly
+                    uint32_t slot;
+                    if (claim<kBatch>(g, x, y, z, epoch, op_mask, op_index, &slot))
+                        g.touched[atomicAdd(&g.ctrl->touched_count[par], 1u)] = slot;
+                }
+            }
+        } else if (box) {
+            for (int x = x0; x <= x1; ++x)
+                for (int y = y0; y <= y1; ++y)
+                    for (int z = z0; z <= z1; ++z) {
+                        if (!lds_insert(lset, pack_coord(x, y, z))) {  // set full: claim directly
+                            uint32_t slot;
+                            if (claim<kBatch>(g, x, y, z, epoch, op_mask, op_index, &slot))
+                                g.touched[atomicAdd(&g.ctrl->touched_count[par], 1u)] = slot;
+                        }
+                    }
+        }
     }
     nvalid = wave_sum(nvalid);
     if (__lane_id() == 0 && nvalid)
