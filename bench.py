@@ -213,8 +213,8 @@ def reintegration(cfg, prims, poses64, keyframes, dev):
     # launch shapes of the batch picked by the autotuner on this workload
     tuned, table = autotune.tune_reintegration(fresh, kds, old, new)
     a, b = fresh(), fresh()
-    b.reserve_batch(2 * keyframes)  # set-up, like the pool itself: the timed batch allocates nothing
     b.set_tuning(**tuned)
+    b.reserve_batch(2 * keyframes)  # set-up, like the pool itself: the timed batch allocates nothing
     torch.cuda.synchronize()
     # plain: the integrate.cpp:108-125 loop, one deIntegrate + integrate per keyframe
     t0 = time.perf_counter()
@@ -237,8 +237,8 @@ def reintegration(cfg, prims, poses64, keyframes, dev):
     t_batch = time.perf_counter() - t0
     # the same batch once more on a fresh grid with per-kernel events
     p = fresh()
-    p.reserve_batch(2 * keyframes)
     p.set_tuning(**tuned)
+    p.reserve_batch(2 * keyframes)
     p.profile(True)
     p.profile_read(reset=True)
     p.reintegrate_batch(kds, old, new)

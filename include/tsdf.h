@@ -111,6 +111,8 @@ typedef struct {
     int32_t integrate_wg_per_cu;  /* brick-update workgroups per CU            */
     int32_t batch_ops_per_pass;   /* ops per collection + update pass of a     */
                                   /* batch, 1..256                              */
+    int32_t batch_graph;          /* 1: a batch of <= 256 ops runs as one       */
+                                  /* captured graph (staging copy + passes)     */
 } tsdf_tuning;
 
 /* Camera of a render view (the differentiable render has its own resolution
@@ -169,8 +171,10 @@ tsdf_status tsdf_reintegrate_batch(tsdf_grid *g, int32_t n, const uint16_t *cons
 
 /* Allocates the device buffers of a batch of up to n_ops ops (per-slot op
  * masks and one point map per op of a pass, at most 256 maps), so that later
- * batches of at most n_ops ops allocate nothing.  Without it the first batch
- * allocates.  No reference counterpart (the reference has no batch). */
+ * batches of at most n_ops ops allocate nothing, and (with the tuning field
+ * batch_graph) captures the graph of a batch of exactly n_ops under the
+ * current tuning.  Without it the first batch allocates.  No reference
+ * counterpart (the reference has no batch). */
 tsdf_status tsdf_reserve_batch(tsdf_grid *g, int32_t n_ops);
 
 /* GPUVoxelGrid::raycast(space, extr), GPUVoxelGrid.cpp:139-147 with

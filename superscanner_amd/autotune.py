@@ -9,6 +9,7 @@ The tunables are the fields of ``tsdf_tuning`` (include/tsdf.h):
                         pass's point maps, ops * H * W * 16 B, should stay in
                         the last-level cache; more ops per pass means fewer
                         payload round trips)
+  batch_graph           a batch of <= 256 ops as one captured graph
   raycast_xcd_map       XCD-banded tile order of the raycast
   raycast_superblocks   certified empty-superblock skipping in the raycast
 
@@ -54,8 +55,8 @@ def tune_reintegration(make_grid, depths, old_poses, new_poses, *, ops_per_pass=
         for _ in range(repeats):
             g = make_grid()
             try:
-                g.reserve_batch(n_ops)
                 g.set_tuning(batch_ops_per_pass=per, integrate_wg_per_cu=wg)
+                g.reserve_batch(n_ops)
                 torch.cuda.synchronize()
                 best = min(best, _timed(lambda: g.reintegrate_batch(depths, old_poses, new_poses)))
             finally:

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -58,11 +59,28 @@ struct Offsets {
 };
 
 constexpr int kThreads = 256;
-constexpr int kTileW = 64;      // collection tile: 64 x 16 pixels, 4 per thread
-constexpr int kTileH = 16;
-constexpr int kLdsSet = 2048;   // per-tile block set in LDS (16 KiB)
+#ifndef TSDF_TILE_W
+#define TSDF_TILE_W 64
+#endif
+#ifndef TSDF_TILE_H
+#define TSDF_TILE_H 16
+#endif
+#ifndef TSDF_LDS_SET
+#define TSDF_LDS_SET 2048
+#endif
+constexpr int kTileW = TSDF_TILE_W;   // collection tile: 64 x 16 pixels = 16 patches of 8 x 8
+constexpr int kTileH = TSDF_TILE_H;
+constexpr int kLdsSet = TSDF_LDS_SET; // per-tile block set in LDS (2048 x 8 B = 16 KiB)
 constexpr int kMaxOps = 256;    // ops per batched pass
 constexpr int kMaskWords = kMaxOps / 32;
+
+// What the host stages for one batch call (pinned copy -> device copy): the
+// first call stamp of the batch and its op descriptors.
+struct BatchStage {
+    uint32_t epoch_base;
+    uint32_t pad[3];
+    OpDesc ops[kMaxOps];
+};
 constexpr int kMaxAxisBlocks = 64;
 
 // ---------------------------------------------------------------------------
@@ -158,15 +176,30 @@ template <bool kBatch>
 __device__ __forceinline__ bool claim(const GridView& g, int x, int y, int z, uint32_t epoch,
                                       uint32_t* op_mask, int op, uint32_t* slot_out)
 {
-    const int s = find_or_insert(g, prime_hash(x, y, z), x, y, z);
-    if (s < 0) {
-        atomicAdd(&g.ctrl->overflow, 1u);
-        return false;
+    // The key's home bucket and its stamp are read together: at the table's
+    // load factor the key is usually there, and then a block that another
+    // tile already claimed costs one round trip.
+    const uint32_t key = prime_hash(x, y, z);
+    const uint32_t b0 = mix32(key) & g.mask;
+    const unsigned long long s0 = __hip_atomic_load(&g.slots[b0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t e0 = __hip_atomic_load(&g.slot_epoch[b0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int s;
+    uint32_t pre;
+    if (s0 != kSlotEmpty && (uint32_t)(s0 >> 32) == key) {
+        s = (int)b0;
+        pre = e0;
+    } else {
+        s = find_or_insert(g, key, x, y, z);
+        if (s < 0) {
+            atomicAdd(&g.ctrl->overflow, 1u);
+            return false;
+        }
+        pre = __hip_atomic_load(&g.slot_epoch[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (kBatch)
         atomicOr(&op_mask[(size_t)s * kMaskWords + (op >> 5)], 1u << (op & 31));
     *slot_out = (uint32_t)s;
-    if (__hip_atomic_load(&g.slot_epoch[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+    if (pre == epoch)
         return false;  // another tile of this call claimed it already
     return atomicExch(&g.slot_epoch[s], epoch) != epoch;
 }
@@ -403,10 +436,14 @@ __global__ __launch_bounds__(kThreads) void k_collect(GridView g, OpDesc op, uin
     collect_tile<false>(g, op, epoch, par, nullptr, 0);
 }
 
-__global__ __launch_bounds__(kThreads) void k_collect_batch(GridView g, const OpDesc* ops, uint32_t epoch,
-                                                            int par, uint32_t* op_mask)
+// A batch pass reads its call stamp from device memory (*epoch_base + pass)
+// so that a captured graph of the whole batch can be replayed with new
+// stamps.
+__global__ __launch_bounds__(kThreads) void k_collect_batch(GridView g, const OpDesc* ops,
+                                                            const uint32_t* epoch_base, int pass, uint32_t* op_mask)
 {
-    collect_tile<true>(g, ops[blockIdx.z], epoch, par, op_mask, blockIdx.z);
+    const uint32_t epoch = *epoch_base + (uint32_t)pass;
+    collect_tile<true>(g, ops[blockIdx.z], epoch, (int)(epoch & 1u), op_mask, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -694,9 +731,11 @@ __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, i
 // equals the sequence of plain calls bit for bit.
 // ---------------------------------------------------------------------------
 template <typename T, bool kColor>
-__global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const OpDesc* ops, int par,
-                                                              Offsets off, uint32_t* op_mask, int n_words)
+__global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const OpDesc* ops,
+                                                              const uint32_t* epoch_base, int pass, Offsets off,
+                                                              uint32_t* op_mask, int n_words)
 {
+    const int par = (int)((*epoch_base + (uint32_t)pass) & 1u);
     const unsigned int count = g.ctrl->touched_count[par];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         g.ctrl->touched_count[par ^ 1] = 0u;
@@ -882,8 +921,12 @@ struct tsdf_grid {
     // batch buffers (allocated on first batched call)
     uint32_t* op_mask;
     float4* ptmap_batch;
-    OpDesc* ops_dev;
-    OpDesc* ops_host;  // pinned
+    BatchStage* stage_dev;
+    BatchStage* stage_host;  // pinned
+    // the batch as one captured graph, rebuilt when its shape changes
+    hipStream_t cap_stream;
+    hipGraphExec_t batch_exec;
+    int batch_key[5];
     int batch_cap;     // point maps allocated in ptmap_batch
     hipEvent_t staging_done;  // recorded after the last copy out of ops_host
     bool staged;
@@ -1025,16 +1068,17 @@ void launch_integrate(const tsdf_grid* g, const OpDesc& op, int par, hipStream_t
 }
 
 template <typename T>
-void launch_integrate_batch(const tsdf_grid* g, const OpDesc* ops, int n_ops, int par, hipStream_t s, bool color)
+void launch_integrate_batch(const tsdf_grid* g, const OpDesc* ops, int n_ops, const uint32_t* epoch_base, int pass,
+                            hipStream_t s, bool color)
 {
     const int grid = persistent_grid(g);
     const int words = (n_ops + 31) / 32;
     if (color)
-        hipLaunchKernelGGL((k_integrate_batch<T, true>), dim3(grid), dim3(kThreads), 0, s, view(g), ops, par,
-                           g->off, g->op_mask, words);
+        hipLaunchKernelGGL((k_integrate_batch<T, true>), dim3(grid), dim3(kThreads), 0, s, view(g), ops, epoch_base,
+                           pass, g->off, g->op_mask, words);
     else
-        hipLaunchKernelGGL((k_integrate_batch<T, false>), dim3(grid), dim3(kThreads), 0, s, view(g), ops, par,
-                           g->off, g->op_mask, words);
+        hipLaunchKernelGGL((k_integrate_batch<T, false>), dim3(grid), dim3(kThreads), 0, s, view(g), ops,
+                           epoch_base, pass, g->off, g->op_mask, words);
 }
 
 tsdf_status check_image_args(const tsdf_grid* g, const uint16_t* depth, int32_t pitch)
@@ -1055,8 +1099,8 @@ tsdf_status ensure_batch_buffers(tsdf_grid* g, int n_ops)
         const size_t nb = g->p.n_buckets;
         HIPC(hipMalloc(&g->op_mask, nb * kMaskWords * sizeof(uint32_t)));
         HIPC(hipMemset(g->op_mask, 0, nb * kMaskWords * sizeof(uint32_t)));
-        HIPC(hipMalloc(&g->ops_dev, kMaxOps * sizeof(OpDesc)));
-        HIPC(hipHostMalloc(&g->ops_host, kMaxOps * sizeof(OpDesc), hipHostMallocDefault));
+        HIPC(hipMalloc(&g->stage_dev, sizeof(BatchStage)));
+        HIPC(hipHostMalloc(&g->stage_host, sizeof(BatchStage), hipHostMallocDefault));
         HIPC(hipEventCreateWithFlags(&g->staging_done, hipEventDisableTiming));
     }
     const int want = n_ops < kMaxOps ? n_ops : kMaxOps;
@@ -1092,9 +1136,9 @@ struct ProfScope {
     tsdf_grid* g;
     hipStream_t s;
     int slot = -1;
-    ProfScope(tsdf_grid* g_, hipStream_t s_, int kind) : g(g_), s(s_)
+    ProfScope(tsdf_grid* g_, hipStream_t s_, int kind, bool enable = true) : g(g_), s(s_)
     {
-        if (!g->prof_on)
+        if (!g->prof_on || !enable)
             return;
         if (g->prof_used == g->prof_cap) {
             const int nc = g->prof_cap ? 2 * g->prof_cap : 1024;
@@ -1324,7 +1368,11 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->ctrl);
     (void)hipFree(g->op_mask);
     (void)hipFree(g->ptmap_batch);
-    (void)hipFree(g->ops_dev);
+    if (g->batch_exec)
+        (void)hipGraphExecDestroy(g->batch_exec);
+    if (g->cap_stream)
+        (void)hipStreamDestroy(g->cap_stream);
+    (void)hipFree(g->stage_dev);
     (void)hipFree(g->fp_bits);
     (void)hipFree(g->fp_keys);
     (void)hipFree(g->fp_counts);
@@ -1340,8 +1388,8 @@ void tsdf_destroy(tsdf_grid* g)
     if (g->render_done)
         (void)hipEventDestroy(g->render_done);
     (void)hipFree(g->view_dev);
-    if (g->ops_host)
-        (void)hipHostFree(g->ops_host);
+    if (g->stage_host)
+        (void)hipHostFree(g->stage_host);
     if (g->staging_done)
         (void)hipEventDestroy(g->staging_done);
     delete g;
@@ -1420,6 +1468,69 @@ tsdf_status tsdf_integrate(tsdf_grid* g, const uint16_t* depth, int32_t pitch, c
     return TSDF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+size_t stage_bytes(int n) { return offsetof(BatchStage, ops) + (size_t)n * sizeof(OpDesc); }
+
+// The passes of a batch of n staged ops on stream s.
+void enqueue_batch_passes(tsdf_grid* g, int n_ops, int per, hipStream_t s, bool profile)
+{
+    const bool color = g->p.with_color != 0;
+    const uint32_t* ep = &g->stage_dev->epoch_base;
+    for (int cb = 0, pass = 0; cb < n_ops; cb += per, ++pass) {
+        const int n = (n_ops - cb) < per ? (n_ops - cb) : per;
+        const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH, n);
+        const OpDesc* ops = g->stage_dev->ops + cb;
+        {
+            ProfScope ps(g, s, TSDF_K_COLLECT_BATCH, profile);
+            hipLaunchKernelGGL(k_collect_batch, cgrid, dim3(kThreads), 0, s, view(g), ops, ep, pass, g->op_mask);
+        }
+        {
+            ProfScope ps(g, s, TSDF_K_INTEGRATE_BATCH, profile);
+            if (g->p.storage == TSDF_STORAGE_F16)
+                launch_integrate_batch<__half>(g, ops, n, ep, pass, s, color);
+            else
+                launch_integrate_batch<float>(g, ops, n, ep, pass, s, color);
+        }
+    }
+}
+
+// Captures (staging copy + passes) for n ops in passes of `per` into one
+// executable graph, unless the current one already has that shape.
+tsdf_status ensure_batch_graph(tsdf_grid* g, int n_ops, int per)
+{
+    const int key[5] = {n_ops, per, persistent_grid(g), g->p.storage, g->p.with_color};
+    if (g->batch_exec && memcmp(key, g->batch_key, sizeof(key)) == 0)
+        return TSDF_OK;
+    if (g->batch_exec) {
+        HIPC(hipGraphExecDestroy(g->batch_exec));
+        g->batch_exec = nullptr;
+    }
+    if (!g->cap_stream)
+        HIPC(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    HIPC(hipStreamBeginCapture(g->cap_stream, hipStreamCaptureModeThreadLocal));
+    (void)hipMemcpyAsync(g->stage_dev, g->stage_host, stage_bytes(n_ops), hipMemcpyHostToDevice, g->cap_stream);
+    enqueue_batch_passes(g, n_ops, per, g->cap_stream, false);
+    const hipError_t ce = hipStreamEndCapture(g->cap_stream, &graph);
+    if (ce != hipSuccess || !graph)
+        return fail(TSDF_HIP_ERROR, std::string("batch graph capture: ") + hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiateWithFlags(&g->batch_exec, graph, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        g->batch_exec = nullptr;
+        return fail(TSDF_HIP_ERROR, std::string("batch graph instantiate: ") + hipGetErrorString(ie));
+    }
+    memcpy(g->batch_key, key, sizeof(key));
+    return TSDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* const* depths, int32_t pitch,
                                  const uint8_t* const* rgbs, const float* E, const int32_t* signs, void* stream)
 {
@@ -1456,8 +1567,9 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
         // the staging buffer may still feed the previous copy
         if (g->staged)
             HIPC(hipEventSynchronize(g->staging_done));
+        BatchStage* st = g->stage_host;
         for (int k = 0; k < sn; ++k) {
-            OpDesc& op = g->ops_host[k];
+            OpDesc& op = st->ops[k];
             op.depth = depths[sbase + k];
             op.rgb = (color && rgbs) ? rgbs[sbase + k] : nullptr;
             op.ptmap = g->ptmap_batch + (size_t)(k % per) * frame;
@@ -1465,28 +1577,23 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
             op.sign = signs[sbase + k];
             make_pose(E + 16 * (size_t)(sbase + k), &op.pose);
         }
-        HIPC(hipMemcpyAsync(g->ops_dev, g->ops_host, sn * sizeof(OpDesc), hipMemcpyHostToDevice, s));
+        const int passes = (sn + per - 1) / per;
+        st->epoch_base = g->epoch + 1;
+        g->epoch += (uint32_t)passes;
+        // one captured graph (staging copy + all passes) when the shape
+        // allows; per-kernel profiling needs the separate launches
+        if (g->tune.batch_graph && !g->prof_on && n_ops <= kMaxOps) {
+            tsdf_status r = ensure_batch_graph(g, sn, per);
+            if (r != TSDF_OK)
+                return r;
+            HIPC(hipGraphLaunch(g->batch_exec, s));
+        } else {
+            HIPC(hipMemcpyAsync(g->stage_dev, st, stage_bytes(sn), hipMemcpyHostToDevice, s));
+            enqueue_batch_passes(g, sn, per, s, true);
+        }
         HIPC(hipEventRecord(g->staging_done, s));
         g->staged = true;
-        for (int cb = 0; cb < sn; cb += per) {
-            const int n = (sn - cb) < per ? (sn - cb) : per;
-            const uint32_t epoch = ++g->epoch;
-            const int par = (int)(epoch & 1u);
-            const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH, n);
-            {
-                ProfScope ps(g, s, TSDF_K_COLLECT_BATCH);
-                hipLaunchKernelGGL(k_collect_batch, cgrid, dim3(kThreads), 0, s, view(g), g->ops_dev + cb, epoch,
-                                   par, g->op_mask);
-            }
-            {
-                ProfScope ps(g, s, TSDF_K_INTEGRATE_BATCH);
-                if (g->p.storage == TSDF_STORAGE_F16)
-                    launch_integrate_batch<__half>(g, g->ops_dev + cb, n, par, s, color);
-                else
-                    launch_integrate_batch<float>(g, g->ops_dev + cb, n, par, s, color);
-            }
-            HIPC(hipGetLastError());
-        }
+        HIPC(hipGetLastError());
     }
     return TSDF_OK;
 }
@@ -1498,7 +1605,13 @@ tsdf_status tsdf_reserve_batch(tsdf_grid* g, int32_t n_ops)
     if (n_ops == 0)
         return TSDF_OK;
     DeviceScope ds(g->p.device);
-    return ensure_batch_buffers(g, n_ops);
+    tsdf_status r = ensure_batch_buffers(g, n_ops);
+    if (r != TSDF_OK || !g->tune.batch_graph || n_ops > kMaxOps)
+        return r;
+    // capture the graph of a batch of exactly n_ops now, so that such a
+    // batch neither allocates nor instantiates
+    const int per = g->tune.batch_ops_per_pass < g->batch_cap ? g->tune.batch_ops_per_pass : g->batch_cap;
+    return ensure_batch_graph(g, n_ops, per);
 }
 
 tsdf_status tsdf_reintegrate_batch(tsdf_grid* g, int32_t n, const uint16_t* const* depths, int32_t pitch,
@@ -1603,6 +1716,7 @@ void tsdf_default_tuning(tsdf_tuning* t)
     t->raycast_superblocks = 0;
     t->integrate_wg_per_cu = 4;
     t->batch_ops_per_pass = 16;
+    t->batch_graph = 1;
 }
 
 tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
@@ -1611,7 +1725,8 @@ tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
         return fail(TSDF_BAD_ARG, "NULL grid or tuning");
     if (t->raycast_xcd_map < 0 || t->raycast_xcd_map > 1 || t->raycast_superblocks < 0 ||
         t->raycast_superblocks > 1 || t->integrate_wg_per_cu < 1 ||
-        t->integrate_wg_per_cu > 32 || t->batch_ops_per_pass < 1 || t->batch_ops_per_pass > kMaxOps)
+        t->integrate_wg_per_cu > 32 || t->batch_ops_per_pass < 1 || t->batch_ops_per_pass > kMaxOps ||
+        t->batch_graph < 0 || t->batch_graph > 1)
         return fail(TSDF_BAD_ARG, "workgroups per CU must lie in [1, 32]");
     g->tune = *t;
     return TSDF_OK;

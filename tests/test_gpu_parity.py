@@ -176,19 +176,20 @@ def test_batched_reintegration_equals_plain_calls(gpu):
     assert_raycast_equal(b, o, new[4])
 
 
-@pytest.mark.parametrize("per,wg", [(1, 4), (3, 2), (4, 8), (64, 3)])
-def test_batch_passes_equal_plain_calls(gpu, per, wg):
+@pytest.mark.parametrize("per,wg,graph", [(1, 4, 1), (3, 2, 1), (4, 8, 0), (4, 8, 1), (64, 3, 1)])
+def test_batch_passes_equal_plain_calls(gpu, per, wg, graph):
     """The batch split into passes of `per` ops (any size, including ones
     that do not divide the op count) and run at other occupancies still
-    equals the plain call sequence bit for bit; a second batch on the same
-    grid (reusing the staging buffer and point maps) too."""
+    equals the plain call sequence bit for bit, as one captured graph or as
+    separate launches; a second batch on the same grid (a replay of the graph
+    with new call stamps, reusing the staging buffer and point maps) too."""
     cfg, intr, poses, depth = room(8, H=120, W=212)
     rng = np.random.default_rng(40 + per)
     new = [synth.pose_f32(synth.perturb(p.astype(np.float64), rng)) for p in poses]
     newer = [synth.pose_f32(synth.perturb(p.astype(np.float64), rng)) for p in poses]
     a, o = make_pair(cfg["voxel_length"], cfg["truncation"], 120, 212, intr)
     b, _ = make_pair(cfg["voxel_length"], cfg["truncation"], 120, 212, intr)
-    b.set_tuning(batch_ops_per_pass=per, integrate_wg_per_cu=wg)
+    b.set_tuning(batch_ops_per_pass=per, integrate_wg_per_cu=wg, batch_graph=graph)
     for k in range(8):
         a.integrate(None, dev_u16(depth[k]), poses[k])
         b.integrate(None, dev_u16(depth[k]), poses[k])
