@@ -264,6 +264,24 @@ tsdf_status tsdf_stats(tsdf_grid *g, tsdf_stats_out *out, void *stream);
 tsdf_status tsdf_export_bricks(tsdf_grid *g, int32_t *coords, float *sdf, float *w,
                                float *rgb, uint64_t cap, uint64_t *n_out, void *stream);
 
+/* GPUVoxelGrid::cleanMemory(space), GPUVoxelGrid.cpp:185-196 with cleanTag
+ * :785-805: erases every brick whose 512 weights are all zero (the surviving
+ * bricks are compacted and the table rebuilt).  Synchronous; *n_removed (or
+ * NULL) receives the number erased. */
+tsdf_status tsdf_clean(tsdf_grid *g, uint64_t *n_removed, void *stream);
+
+/* Triangle mesh of the zero level set (stands in for marchingCubesHost,
+ * GPUVoxelGrid.cpp:200-251, which hands the bricks to Open3D's
+ * ExtractTriangleMesh): marching tetrahedra over every cell of 8 observed
+ * voxel centres, 6 tetrahedra per cell split around the cell diagonal, each
+ * triangle facing the free-space side.  triangles: DEVICE [cap][3][3] float
+ * world coordinates (a triangle soup); colors: DEVICE [cap][3][3] vertex
+ * colours interpolated along the same edges, or NULL (a grid without colour
+ * accepts only NULL); *n_triangles (host) receives the total, which may
+ * exceed cap (then only cap are written).  Synchronous. */
+tsdf_status tsdf_extract_mesh(tsdf_grid *g, float *triangles, float *colors, uint64_t cap,
+                              uint64_t *n_triangles, void *stream);
+
 /* Host helper: the fp32 pose data every kernel uses -- E^-1 of a rigid pose
  * evaluated in double and rounded once (stand-in for Eigen's float inverse,
  * GPUVoxelGrid.cpp:55) and Ez = normalize(third column) (:52-53). */

@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
     L.orc_n_bricks.restype = C.c_int64
     L.orc_export.argtypes = [vp, vp, vp, vp, vp, vp]
     L.orc_export.restype = None
+    L.orc_clean.argtypes = [vp]
+    L.orc_clean.restype = C.c_int64
     L.orc_set_mutation.argtypes = [C.c_int]
     L.orc_set_mutation.restype = None
     dp = P(C.c_double)
@@ -217,6 +219,10 @@ class OracleGrid:
 
     def n_bricks(self) -> int:
         return int(lib().orc_n_bricks(self._h))
+
+    def clean(self) -> int:
+        """cleanMemory: erases all-zero-weight bricks; returns the count."""
+        return int(lib().orc_clean(self._h))
 
     def export(self):
         n = self.n_bricks()

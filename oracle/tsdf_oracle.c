@@ -723,6 +723,39 @@ void orc_raycast(orc_grid *g, const float E[16], float *points, float *colors, f
 
 int64_t orc_n_bricks(const orc_grid *g) { return g->n; }
 
+/* cleanMemory / cleanTag, GPUVoxelGrid.cpp:185-196, 785-805: every block
+ * whose 512 weights are all zero is erased from the map.  Surviving bricks
+ * keep their relative order (brick indices are not observable). */
+int64_t orc_clean(orc_grid *g)
+{
+    int64_t kept = 0;
+    for (int64_t b = 0; b < g->n; b++) {
+        int empty = 1;
+        for (int v = 0; v < VPB && empty; v++)
+            if (g->w[b * VPB + v] != 0.0f)
+                empty = 0;
+        if (empty)
+            continue;
+        if (kept != b) {
+            g->coord[kept] = g->coord[b];
+            g->key[kept] = g->key[b];
+            g->stamp[kept] = g->stamp[b];
+            memmove(g->sdf + kept * VPB, g->sdf + b * VPB, VPB * sizeof(float));
+            memmove(g->w + kept * VPB, g->w + b * VPB, VPB * sizeof(float));
+            if (g->with_color)
+                memmove(g->rgb + kept * VPB * 3, g->rgb + b * VPB * 3, VPB * 3 * sizeof(float));
+        }
+        kept++;
+    }
+    int64_t removed = g->n - kept;
+    g->n = kept;
+    for (uint64_t i = 0; i < g->hcap; i++)
+        g->hval[i] = -1;
+    for (int64_t b = 0; b < g->n; b++)
+        h_put(g, (uint32_t)g->key[b], b);
+    return removed;
+}
+
 void orc_export(const orc_grid *g, int32_t *coords, int64_t *keys, float *sdf, float *w, float *rgb)
 {
     for (int64_t b = 0; b < g->n; b++) {

@@ -35,6 +35,7 @@ EXPORTS = (
     "tsdf_last_error", "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
     "tsdf_default_tuning", "tsdf_get_tuning", "tsdf_set_tuning",
     "tsdf_render", "tsdf_render_backward", "tsdf_track_normal_eq", "tsdf_se3_gauss_newton_step",
+    "tsdf_clean", "tsdf_extract_mesh",
 )
 
 KERNEL_KINDS = ("reset", "collect", "integrate", "collect_batch", "integrate_batch", "raycast")
@@ -127,6 +128,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_render_backward.argtypes = [vp, C.c_int32, fp, P(TsdfView), vp, vp, vp, vp, vp, vp]
     L.tsdf_track_normal_eq.argtypes = [vp, C.c_int32, fp, P(TsdfView), vp, C.c_float, vp, vp, vp]
     L.tsdf_se3_gauss_newton_step.argtypes = [dp, dp, dp, dp, dp]
+    L.tsdf_clean.argtypes = [vp, P(C.c_uint64), vp]
+    L.tsdf_extract_mesh.argtypes = [vp, vp, vp, C.c_uint64, P(C.c_uint64), vp]
     L.tsdf_raycast.argtypes = [vp, fp, vp, vp, vp, vp, vp]
     L.tsdf_sync.argtypes = [vp, vp]
     L.tsdf_stats.argtypes = [vp, P(TsdfStats), vp]

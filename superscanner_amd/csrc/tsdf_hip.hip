@@ -1832,3 +1832,4 @@ tsdf_status tsdf_export_bricks(tsdf_grid* g, int32_t* coords, float* sdf, float*
 }  // extern "C"
 
 #include "tsdf_render.hpp"
+#include "tsdf_maintain.hpp"

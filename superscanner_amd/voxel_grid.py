@@ -261,6 +261,51 @@ class GPUVoxelGrid:
         """voxelMap2.size()"""
         return self.stats()["n_bricks"]
 
+    def cleanMemory(self, stream=None) -> int:
+        """GPUVoxelGrid::cleanMemory (GPUVoxelGrid.cpp:185-196): erases the
+        bricks whose weights are all zero; returns how many."""
+        n = C.c_uint64(0)
+        check(self._L.tsdf_clean(self._h, C.byref(n), _stream_ptr(stream)), "tsdf_clean")
+        return int(n.value)
+
+    def extract_mesh(self, stream=None) -> dict:
+        """Mesh of the zero level set (stands in for marchingCubesHost,
+        GPUVoxelGrid.cpp:200-251, and the ComputeVertexNormals that follows):
+        {"triangles": [m, 3, 3] device float32 soup, "triangle_colors":
+        [m, 3, 3] device float32 or None, "vertices": [k, 3] host float32
+        merged by exact coordinates, "faces": [m, 3] host int64,
+        "normals": [k, 3] host float32 unit area-weighted vertex normals,
+        "colors": [k, 3] host float32 or None}."""
+        cap = max(1024, 64 * self.size())
+        while True:
+            tris = torch.empty((cap, 3, 3), dtype=torch.float32, device=self.device)
+            cols = torch.empty((cap, 3, 3), dtype=torch.float32, device=self.device) if self.withColor else None
+            n = C.c_uint64(0)
+            check(self._L.tsdf_extract_mesh(self._h, C.c_void_p(tris.data_ptr()),
+                                            C.c_void_p(cols.data_ptr()) if cols is not None else None, cap,
+                                            C.byref(n), _stream_ptr(stream)), "tsdf_extract_mesh")
+            if n.value <= cap:
+                break
+            cap = int(n.value)
+        m = int(n.value)
+        tris = tris[:m]
+        cols = cols[:m] if cols is not None else None
+        soup = tris.reshape(-1, 3).cpu().numpy()
+        verts, first, inv = np.unique(soup.view(np.uint32).reshape(-1, 3), axis=0, return_index=True,
+                                      return_inverse=True)
+        verts = verts.view(np.float32).reshape(-1, 3)
+        faces = inv.reshape(-1, 3).astype(np.int64)
+        t = soup.reshape(-1, 3, 3).astype(np.float64)
+        fn = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+        vn = np.zeros((len(verts), 3))
+        for q in range(3):
+            np.add.at(vn, faces[:, q], fn)
+        norm = np.linalg.norm(vn, axis=1, keepdims=True)
+        vn = (vn / np.where(norm > 0, norm, 1.0)).astype(np.float32)
+        vcol = cols.reshape(-1, 3).cpu().numpy()[first] if cols is not None else None
+        return {"triangles": tris, "triangle_colors": cols, "vertices": verts, "faces": faces, "normals": vn,
+                "colors": vcol}
+
     def sdf_tensor(self) -> torch.Tensor:
         """The stored sdf values as a device float32 tensor [n_bricks, 512]
         (row = brick id, voxel x + 8y + 64z): the voxel parameters of the
