@@ -282,6 +282,66 @@ tsdf_status tsdf_clean(tsdf_grid *g, uint64_t *n_removed, void *stream);
 tsdf_status tsdf_extract_mesh(tsdf_grid *g, float *triangles, float *colors, uint64_t cap,
                               uint64_t *n_triangles, void *stream);
 
+/* ---- point maps, normal maps and projective ICP (kokkosICP.cpp) ---------- */
+
+typedef enum {
+    TSDF_ICP_POINT_TO_POINT = 0, /* ICPTypes::PointToPoint, kokkosICP.h:4-8 */
+    TSDF_ICP_POINT_TO_PLANE = 1  /* ICPTypes::PointToPlane                  */
+} tsdf_icp_type;
+
+typedef struct {
+    int32_t type;         /* tsdf_icp_type                                   */
+    int32_t n_iterations; /* n_it = 10, kokkosICP.cpp:49                     */
+    int32_t n_search;     /* window half-width, n_search = 5, :48 (<= 64)    */
+    float max_dist;       /* metres, max_dist = 0.075, :50                   */
+} tsdf_icp_params;
+
+/* Number of sums tsdf_icp_moments returns. */
+#define TSDF_ICP_MOMENTS 18
+
+void tsdf_icp_default_params(tsdf_icp_params *p);
+
+/* CreatePcdFromDepthView, imageutil.cpp:89-111: points DEVICE [H][W][3] float
+ * = ((j - cx) / fx d, (i - (H - cy)) / fy d, d), d = depth / 1000 (depth
+ * DEVICE [H][W] uint16 mm, row-major).  Asynchronous on the stream. */
+tsdf_status tsdf_depth_to_points(const uint16_t *depth, const tsdf_view *cam, float *points, void *stream);
+
+/* CalculateNormalsFromPCDView, imageutil.cpp:116-157: normals DEVICE
+ * [H][W][3] float; (0, 0, 1) on the border and where the pixel or a
+ * 4-neighbour has z == 0, else the normalised sum of the four neighbour cross
+ * products.  Asynchronous. */
+tsdf_status tsdf_points_to_normals(const float *points, int32_t height, int32_t width, float *normals,
+                                   void *stream);
+
+/* The "Closest Point Kernel" of kokkosICP, kokkosICP.cpp:69-115: for every
+ * pixel of x (z != 0) the closest point of y in the 2n x 2n window around the
+ * projection of Txy x (row-major float Txy, the float cast the reference
+ * uses), or its projection onto the tangent plane of y (point-to-plane, with
+ * y_normals); x_nearest DEVICE [H][W][3], (0, 0, 0) where none qualifies.
+ * Asynchronous. */
+tsdf_status tsdf_icp_associate(const float *x_points, const float *y_points, const float *y_normals,
+                               const float Txy[16], const tsdf_view *cam, const tsdf_icp_params *p,
+                               float *x_nearest, void *stream);
+
+/* The sums of one ICP iteration (kokkosICP.cpp:117-171), in double:
+ * [0] points of x, [1] correspondences, [2..4] sum x, [5..7] sum x_nearest,
+ * [8..16] sum x x_nearest^T (row-major), [17] sum |x_nearest - Txy x|^2 with
+ * the double Txy.  moments: HOST [18].  Synchronous. */
+tsdf_status tsdf_icp_moments(const float *x_points, const float *x_nearest, const double Txy[16],
+                             int32_t height, int32_t width, double *moments, void *stream);
+
+/* kokkosICP(space, d1, d2, tx, ty, type, &rsme), kokkosICP.cpp:23-191:
+ * aligns the points of depth1 (pose tx, camera->world) to those of depth2
+ * (pose ty), starting from Txy = ty^-1 tx; each iteration associates, takes
+ * the centroids and the rotation of the SVD of the centred cross-covariance,
+ * and the transform of the iteration with the lowest RMSE is returned in
+ * T_out (host, row-major, x -> y); *rmse_out that RMSE (1e6 when no iteration
+ * found a correspondence) and *fitness_out the best correspondences / points
+ * ratio (either may be NULL).  Synchronous. */
+tsdf_status tsdf_icp(const uint16_t *depth1, const uint16_t *depth2, const double tx[16], const double ty[16],
+                     const tsdf_view *cam, const tsdf_icp_params *p, double T_out[16], float *rmse_out,
+                     float *fitness_out, void *stream);
+
 /* Host helper: the fp32 pose data every kernel uses -- E^-1 of a rigid pose
  * evaluated in double and rounded once (stand-in for Eigen's float inverse,
  * GPUVoxelGrid.cpp:55) and Ez = normalize(third column) (:52-53). */

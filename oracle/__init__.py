@@ -108,6 +108,15 @@ def lib() -> C.CDLL:
     L.orc64_n_bricks.restype = C.c_int64
     L.orc64_export.argtypes = [vp, vp, vp, vp]
     L.orc64_export.restype = None
+    L.orc_depth_to_points.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp]
+    L.orc_depth_to_points.restype = None
+    L.orc_normals.argtypes = [vp, C.c_int, C.c_int, vp]
+    L.orc_normals.restype = None
+    L.orc_icp_associate.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_int, C.c_int, C.c_float, vp]
+    L.orc_icp_associate.restype = None
+    L.orc_icp_sums.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
+    L.orc_icp_sums.restype = None
     _lib = L
     return L
 

@@ -125,6 +125,14 @@ int64_t  orc64_n_bricks(const orc64_grid *g);
 /* coords n*3 int32, sdf / w n*512 double (voxel x + 8y + 64z). */
 void     orc64_export(const orc64_grid *g, int32_t *coords, double *sdf, double *w);
 
+/* icp_oracle.c: point maps, normal maps, projective ICP (imageutil.cpp,
+ * kokkosICP.cpp) */
+void orc_depth_to_points(const uint16_t *depth, int H, int W, float fx, float fy, float cx, float cy, float *out);
+void orc_normals(const float *pcd, int H, int W, float *out);
+void orc_icp_associate(const float *x, const float *y, const float *yn, const float *T, int H, int W, float fx,
+                       float fy, float cx, float cy, int type, int n_search, float max_dist, float *out);
+void orc_icp_sums(const float *x, const float *xn, const double *T, int H, int W, double *out);
+
 #ifdef __cplusplus
 }
 #endif

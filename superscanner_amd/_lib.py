@@ -36,9 +36,24 @@ EXPORTS = (
     "tsdf_default_tuning", "tsdf_get_tuning", "tsdf_set_tuning",
     "tsdf_render", "tsdf_render_backward", "tsdf_track_normal_eq", "tsdf_se3_gauss_newton_step",
     "tsdf_clean", "tsdf_extract_mesh",
+    "tsdf_icp_default_params", "tsdf_depth_to_points", "tsdf_points_to_normals", "tsdf_icp_associate",
+    "tsdf_icp_moments", "tsdf_icp",
 )
 
+ICP_POINT_TO_POINT = 0
+ICP_POINT_TO_PLANE = 1
+ICP_MOMENTS = 18
+
 KERNEL_KINDS = ("reset", "collect", "integrate", "collect_batch", "integrate_batch", "raycast")
+
+
+class TsdfIcpParams(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("n_iterations", C.c_int32),
+        ("n_search", C.c_int32),
+        ("max_dist", C.c_float),
+    ]
 
 
 class TsdfParams(C.Structure):
@@ -130,6 +145,13 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_se3_gauss_newton_step.argtypes = [dp, dp, dp, dp, dp]
     L.tsdf_clean.argtypes = [vp, P(C.c_uint64), vp]
     L.tsdf_extract_mesh.argtypes = [vp, vp, vp, C.c_uint64, P(C.c_uint64), vp]
+    L.tsdf_icp_default_params.argtypes = [P(TsdfIcpParams)]
+    L.tsdf_icp_default_params.restype = None
+    L.tsdf_depth_to_points.argtypes = [vp, P(TsdfView), vp, vp]
+    L.tsdf_points_to_normals.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
+    L.tsdf_icp_associate.argtypes = [vp, vp, vp, fp, P(TsdfView), P(TsdfIcpParams), vp, vp]
+    L.tsdf_icp_moments.argtypes = [vp, vp, dp, C.c_int32, C.c_int32, dp, vp]
+    L.tsdf_icp.argtypes = [vp, vp, dp, dp, P(TsdfView), P(TsdfIcpParams), dp, fp, fp, vp]
     L.tsdf_raycast.argtypes = [vp, fp, vp, vp, vp, vp, vp]
     L.tsdf_sync.argtypes = [vp, vp]
     L.tsdf_stats.argtypes = [vp, P(TsdfStats), vp]
