@@ -342,6 +342,27 @@ tsdf_status tsdf_icp(const uint16_t *depth1, const uint16_t *depth2, const doubl
                      const tsdf_view *cam, const tsdf_icp_params *p, double T_out[16], float *rmse_out,
                      float *fitness_out, void *stream);
 
+/* ---- reintegration scheduler helpers (host only, no device work) ------- */
+
+/* MattoDof, matrixutil.cpp:38-64: (rx, ry, rz, tx, ty, tz) of a row-major
+ * camera->world T with R = Rz(rz) Ry(ry) Rx(rx). */
+tsdf_status tsdf_mat_to_dof(const double T[16], double dof[6]);
+
+/* getT, matrixutil.h:20-63: trans(t) Rz Ry Rx, evaluated left to right. */
+tsdf_status tsdf_dof_to_mat(const double dof[6], double T[16]);
+
+/* diff, integrate.cpp:211-216: |[world - integrated](0..2) , 2 [world -
+ * integrated](3..5)| -- rotation angles, then twice the translation. */
+double tsdf_reint_diff(const double world_dof[6], const double integrated_dof[6]);
+
+/* updateReintegrationBuffer, integrate.cpp:223-234: order [n] receives the
+ * list indices of the integrated frames (world_dof / integrated_dof: [n][6],
+ * in list order) sorted by diff, largest first, equal ones in list order (the
+ * list's new order); the first *n_selected of them, up to the first diff
+ * below treint, form the reintegration buffer. */
+tsdf_status tsdf_reint_select(int32_t n, const double *world_dof, const double *integrated_dof, double treint,
+                              int32_t *order, int32_t *n_selected);
+
 /* Host helper: the fp32 pose data every kernel uses -- E^-1 of a rigid pose
  * evaluated in double and rounded once (stand-in for Eigen's float inverse,
  * GPUVoxelGrid.cpp:55) and Ez = normalize(third column) (:52-53). */

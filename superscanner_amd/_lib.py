@@ -38,6 +38,7 @@ EXPORTS = (
     "tsdf_clean", "tsdf_extract_mesh",
     "tsdf_icp_default_params", "tsdf_depth_to_points", "tsdf_points_to_normals", "tsdf_icp_associate",
     "tsdf_icp_moments", "tsdf_icp",
+    "tsdf_mat_to_dof", "tsdf_dof_to_mat", "tsdf_reint_diff", "tsdf_reint_select",
 )
 
 ICP_POINT_TO_POINT = 0
@@ -152,6 +153,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_icp_associate.argtypes = [vp, vp, vp, fp, P(TsdfView), P(TsdfIcpParams), vp, vp]
     L.tsdf_icp_moments.argtypes = [vp, vp, dp, C.c_int32, C.c_int32, dp, vp]
     L.tsdf_icp.argtypes = [vp, vp, dp, dp, P(TsdfView), P(TsdfIcpParams), dp, fp, fp, vp]
+    L.tsdf_mat_to_dof.argtypes = [dp, dp]
+    L.tsdf_dof_to_mat.argtypes = [dp, dp]
+    L.tsdf_reint_diff.argtypes = [dp, dp]
+    L.tsdf_reint_diff.restype = C.c_double
+    L.tsdf_reint_select.argtypes = [C.c_int32, dp, dp, C.c_double, P(C.c_int32), P(C.c_int32)]
     L.tsdf_raycast.argtypes = [vp, fp, vp, vp, vp, vp, vp]
     L.tsdf_sync.argtypes = [vp, vp]
     L.tsdf_stats.argtypes = [vp, P(TsdfStats), vp]

@@ -1834,3 +1834,4 @@ tsdf_status tsdf_export_bricks(tsdf_grid* g, int32_t* coords, float* sdf, float*
 #include "tsdf_render.hpp"
 #include "tsdf_maintain.hpp"
 #include "tsdf_icp.hpp"
+#include "tsdf_sched.hpp"
