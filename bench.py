@@ -21,6 +21,11 @@ Besides the headline line the JSON carries:
                 64 keyframes re-fused at perturbed poses (valid fraction
                 U(0.3, 1) per keyframe): one batched pass vs 128 plain calls,
                 with the resulting TSDFs compared for identity
+  maintenance   mesh extraction and cleanMemory on the map of the last step
+  icp           projective association (both types) on two 640x480 frames,
+                HIP-event timed, and a whole kokkosICP call
+  ingest        PNG sequence -> pinned host -> HBM frame rate, and the pinned
+                host -> device copy rate (PCIe-inclusive, not the headline)
 """
 from __future__ import annotations
 
@@ -54,6 +59,9 @@ def parse():
     ap.add_argument("--no-reintegration", action="store_true")
     ap.add_argument("--keyframes", type=int, default=64)
     ap.add_argument("--no-diff-render", action="store_true")
+    ap.add_argument("--no-maintenance", action="store_true")
+    ap.add_argument("--no-icp", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--fp16-frames", type=int, default=2000)
     ap.add_argument("--footprint-samples", type=int, default=12)
@@ -394,6 +402,139 @@ CAL_BYTES = 256 << 20   # calibration copy: 256 MiB read + 256 MiB written
 KERNEL_NAMES = {"collect": "k_collect(", "integrate": "k_integrate<", "raycast": "k_raycast<"}
 
 
+def maintenance_leg(grid):
+    """cleanMemory and mesh extraction on the map the timed steps built."""
+    import torch
+    torch.cuda.synchronize()
+    out = {"bricks": grid.size()}
+    t0 = time.perf_counter()
+    m = grid.extract_mesh()
+    torch.cuda.synchronize()
+    out["mesh_first_call_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    n_tri = int(m["triangles"].shape[0])
+    cap = n_tri + 1
+    tris = torch.empty((cap, 3, 3), dtype=torch.float32, device=grid.device)
+    cnt = C.c_uint64(0)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        grid._L.tsdf_extract_mesh(grid._h, C.c_void_p(tris.data_ptr()), None, None, cap, C.byref(cnt), None)
+    mesh_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        grid.extract_mesh()
+    indexed_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out.update({"triangles": n_tri, "vertices": int(m["vertices"].shape[0]), "mesh_ms": round(mesh_ms, 3),
+                "mesh_Mtri_per_s": round(n_tri / (mesh_ms * 1e-3) / 1e6, 1),
+                "indexed_mesh_ms": round(indexed_ms, 2),
+                "note": "mesh_ms: tsdf_extract_mesh (triangle soup) per call, synchronous; indexed_mesh_ms: "
+                        "GPUVoxelGrid.extract_mesh end to end (soup + keys, welding and vertex normals on the "
+                        "GPU, copy of the indexed mesh to the host)"})
+    t0 = time.perf_counter()
+    removed = grid.cleanMemory()
+    out["clean_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    out["clean_removed"] = removed
+    return out
+
+
+def icp_leg(dev, reps=50):
+    """Projective ICP between two 640x480 frames of the room: the
+    association kernel timed with HIP events on its stream, and the whole
+    kokkosICP (10 iterations) per call."""
+    import torch
+    from superscanner_amd import icp, synth
+    h, w = 480, 640
+    intr = (525.0, 525.0, 319.5, 239.5)
+    room = synth.scene_room(seed=1)
+    T1 = synth.look_at((0.5, 0.3, 1.4), (2.4, 1.9, 0.8))
+    M = np.eye(4)
+    M[:3, :3] = synth.axis_angle((0.3, 1.0, 0.2), np.deg2rad(1.5))
+    M[:3, 3] = (0.01, -0.008, 0.012)
+    T2 = T1 @ M
+    d1 = synth.render_depth(room, T1, h, w, intr, noise_mm=1.0, max_range_m=6.0, seed=1)
+    d2 = synth.render_depth(room, T2, h, w, intr, noise_mm=1.0, max_range_m=6.0, seed=2)
+    x = icp.CreatePcdFromDepthView(d1, intr)
+    y = icp.CreatePcdFromDepthView(d2, intr)
+    yn = icp.CalculateNormalsFromPCDView(y)
+    Txy = np.linalg.inv(T2) @ T1
+    s = torch.cuda.current_stream()
+    out = {"frame": [w, h]}
+    for name, t in (("point_to_point", icp.PointToPoint), ("point_to_plane", icp.PointToPlane)):
+        icp.associate(x, y, yn, Txy, intr, icp_type=t)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(reps):
+            xn = icp.associate(x, y, yn, Txy, intr, icp_type=t)
+        b.record(s)
+        b.synchronize()
+        us = a.elapsed_time(b) * 1e3 / reps
+        # compulsory bytes: x and the output (12 B each), y once (12 B), and
+        # the normals of y once for point to plane
+        per_px = 36 + (12 if t == icp.PointToPlane else 0)
+        gbs = per_px * h * w / (us * 1e-6) / 1e9
+        out[name] = {"associate_us": round(us, 2), "bytes_per_px": per_px, "achieved_GBs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "matched_frac": round(float((xn[..., 2] != 0).float().mean()), 3)}
+    dd1 = torch.from_numpy(d1.view(np.int16)).to(dev)
+    dd2 = torch.from_numpy(d2.view(np.int16)).to(dev)
+    icp.kokkosICP(dd1, dd2, T1, T2, intr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        T, rmse, fit = icp.kokkosICP(dd1, dd2, T1, T2, intr)
+    out["kokkosICP_ms"] = round((time.perf_counter() - t0) * 1e3 / 5, 3)
+    out["kokkosICP_rmse"], out["kokkosICP_fitness"] = round(rmse, 6), round(fit, 4)
+    return out
+
+
+def ingest_leg(dev, n=32, H=480, W=848):
+    """Recorded frames from disk to HBM: 16-bit PNG depth + PNG colour decoded
+    by a thread pool, staged in pinned memory, uploaded on a copy stream; and
+    the pinned host -> device copy rate alone (PCIe-inclusive figures)."""
+    import tempfile
+    import torch
+    from PIL import Image
+    from superscanner_amd import frames as F
+    rng = np.random.default_rng(0)
+    out = {"frames": n, "size": [W, H]}
+    with tempfile.TemporaryDirectory() as root:
+        os.makedirs(os.path.join(root, "depth"))
+        os.makedirs(os.path.join(root, "color"))
+        base = (rng.integers(400, 5000, size=(H, W))).astype(np.uint16)
+        col = rng.integers(0, 256, size=(H, W, 3), dtype=np.uint8)
+        for k in range(n):
+            Image.fromarray(np.roll(base, k, axis=1)).save(os.path.join(root, "depth", F.getPicNumberString(k) + ".png"))
+            Image.fromarray(np.roll(col, k, axis=1)).save(os.path.join(root, "color", F.getPicNumberString(k) + ".png"))
+        F.write_intrinsic(os.path.join(root, "intrinsic.txt"), W, H,
+                          [[430.0, 0, 423.5], [0, 430.0, 239.5], [0, 0, 1]])
+        for with_color in (False, True):
+            rd = F.SequenceReader(root, 0, n, H, W, with_color=with_color, prefetch=8, workers=8)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for f in rd:
+                pass
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            rd.close()
+            key = "depth_and_color" if with_color else "depth_only"
+            out[key] = {"frames_per_s": round(n / dt, 1),
+                        "MB_per_s": round(n * H * W * (5 if with_color else 2) / dt / 1e6, 1)}
+    nbytes = 256 << 20
+    h = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    d = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(5):
+        d.copy_(h, non_blocking=True)
+    b.record()
+    b.synchronize()
+    out["pinned_h2d_GBs"] = round(5 * nbytes / (a.elapsed_time(b) * 1e-3) / 1e9, 1)
+    out["depth_frame_h2d_us"] = round(H * W * 2 / (out["pinned_h2d_GBs"] * 1e9) * 1e6, 1)
+    return out
+
+
 def pmc_workload(args):
     """Two steps of the workload and one plain copy kernel of known size."""
     import torch
@@ -577,6 +718,15 @@ def main():
         "raycast_footprint": {"u_vox": round(u_vox), "u_blk": round(u_blk), "samples": args.footprint_samples},
         "tuning": tuning,
     }
+    if not args.no_maintenance:
+        out["maintenance"] = maintenance_leg(grid)
+        print("[bench] mesh + clean done", file=sys.stderr, flush=True)
+    if not args.no_icp:
+        out["icp"] = icp_leg(dev)
+        print("[bench] icp done", file=sys.stderr, flush=True)
+    if not args.no_ingest:
+        out["ingest"] = ingest_leg(dev)
+        print("[bench] ingest done", file=sys.stderr, flush=True)
     if not args.no_reintegration:
         out["batched_reintegration"] = reintegration(cfg, prims, poses64, args.keyframes, dev)
         print("[bench] reintegration done", file=sys.stderr, flush=True)

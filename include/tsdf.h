@@ -277,9 +277,14 @@ tsdf_status tsdf_clean(tsdf_grid *g, uint64_t *n_removed, void *stream);
  * triangle facing the free-space side.  triangles: DEVICE [cap][3][3] float
  * world coordinates (a triangle soup); colors: DEVICE [cap][3][3] vertex
  * colours interpolated along the same edges, or NULL (a grid without colour
- * accepts only NULL); *n_triangles (host) receives the total, which may
- * exceed cap (then only cap are written).  Synchronous. */
-tsdf_status tsdf_extract_mesh(tsdf_grid *g, float *triangles, float *colors, uint64_t cap,
+ * accepts only NULL); vertex_keys: DEVICE [cap][3] or NULL, a key per vertex
+ * naming its edge (inside voxel's global index, biased by 2^18 per axis, in
+ * bits 43-61 / 24-42 / 5-23, and the offset to the outside voxel,
+ * (dx+1) + 3 (dy+1) + 9 (dz+1), in bits 0-4): equal keys are equal vertices,
+ * which welds the soup into an indexed mesh (maps beyond +-32767 blocks per
+ * axis are refused with keys); *n_triangles (host) receives the total, which
+ * may exceed cap (then only cap are written).  Synchronous. */
+tsdf_status tsdf_extract_mesh(tsdf_grid *g, float *triangles, float *colors, uint64_t *vertex_keys, uint64_t cap,
                               uint64_t *n_triangles, void *stream);
 
 /* ---- point maps, normal maps and projective ICP (kokkosICP.cpp) ---------- */

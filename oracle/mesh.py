@@ -54,17 +54,30 @@ def _edge_col(c, s, a, b):
     return c[:, a] + t[:, None] * (c[:, b] - c[:, a])
 
 
-def extract_mesh(coords, sdf, w, vl, rgb=None):
+def vertex_key(g_in, d):
+    """Key of the vertex on the edge from the inside voxel g_in [.., 3]
+    (global voxel index) to g_in + d: the index biased by 2^18 per axis in
+    bits 43-61 / 24-42 / 5-23 and (dx+1) + 3 (dy+1) + 9 (dz+1) in bits 0-4."""
+    g = np.asarray(g_in, np.int64) + (1 << 18)
+    d = np.asarray(d, np.int64)
+    return (g[..., 0] << 43) | (g[..., 1] << 24) | (g[..., 2] << 5) | ((d[..., 0] + 1) + 3 * (d[..., 1] + 1)
+                                                                        + 9 * (d[..., 2] + 1))
+
+
+def extract_mesh(coords, sdf, w, vl, rgb=None, keys=False):
     """Triangle soup [m, 3, 3] float32 of exported bricks (coords [n, 3],
     sdf / w [n, 512], voxel x + 8y + 64z); with rgb [n, 512, 3] also the
-    vertex colours [m, 3, 3], interpolated along the same edges, as a pair."""
+    vertex colours [m, 3, 3], interpolated along the same edges; with keys
+    the vertex keys [m, 3] (vertex_key).  Several outputs come as a tuple
+    (soup, colours if rgb, keys if keys)."""
     coords = np.asarray(coords, np.int64).reshape(-1, 3)
     sdf = np.asarray(sdf, np.float32).reshape(-1, 512)
     w = np.asarray(w, np.float32).reshape(-1, 512)
     n = coords.shape[0]
     if n == 0:
         z = np.zeros((0, 3, 3), np.float32)
-        return (z, z.copy()) if rgb is not None else z
+        outs = [z] + ([z.copy()] if rgb is not None else []) + ([np.zeros((0, 3), np.int64)] if keys else [])
+        return tuple(outs) if len(outs) > 1 else z
     if rgb is not None:
         rgb = np.asarray(rgb, np.float32).reshape(-1, 512, 3)
     index = {tuple(c): k for k, c in enumerate(coords.tolist())}
@@ -115,7 +128,9 @@ def extract_mesh(coords, sdf, w, vl, rgb=None):
         p[:, c, 2] = pz[B, Z + dz]
         if col is not None:
             col[:, c] = Cp[B, Z + dz, Y + dy, X + dx]
-    out, out_col = [], []
+    out, out_col, out_key = [], [], []
+    gbase = coords[B] * 8 + np.stack([X, Y, Z], 1)  # global index of corner 0 of each cell
+    corner = np.array([[c & 1, (c >> 1) & 1, (c >> 2) & 1] for c in range(8)], np.int64)
     for tet in (TETS[:5] if _mutation == 3 else TETS):
         inside = s[:, list(tet)] < 0
         for case in range(1, 15):
@@ -134,6 +149,9 @@ def extract_mesh(coords, sdf, w, vl, rgb=None):
                 e00, e01, e11, e10 = (ins[0], outs[0]), (ins[0], outs[1]), (ins[1], outs[1]), (ins[1], outs[0])
                 edges = [[e00, e01, e11], [e00, e11, e10]]
             tris = [np.stack([_edge(pp, ss, a, b) for a, b in e], 1) for e in edges]
+            gb = gbase[sel]
+            tkeys = [np.stack([vertex_key(gb + corner[a], corner[b] - corner[a]) for a, b in e], 1)
+                     for e in edges] if keys else None
             cc = col[sel] if col is not None else None
             tcols = [np.stack([_edge_col(cc, ss, a, b) for a, b in e], 1) for e in edges] if cc is not None else None
             d = np.zeros((pp.shape[0], 3), np.float32)
@@ -157,10 +175,17 @@ def extract_mesh(coords, sdf, w, vl, rgb=None):
                     tc = tcols[j].copy()
                     tc[flip, 1], tc[flip, 2] = tc[flip, 2].copy(), tc[flip, 1].copy()
                     out_col.append(tc)
+                if tkeys is not None:
+                    tk = tkeys[j].copy()
+                    tk[flip, 1], tk[flip, 2] = tk[flip, 2].copy(), tk[flip, 1].copy()
+                    out_key.append(tk)
     tris = np.concatenate(out, 0) if out else np.zeros((0, 3, 3), np.float32)
-    if rgb is None:
-        return tris
-    return tris, (np.concatenate(out_col, 0) if out_col else np.zeros((0, 3, 3), np.float32))
+    outs = [tris]
+    if rgb is not None:
+        outs.append(np.concatenate(out_col, 0) if out_col else np.zeros((0, 3, 3), np.float32))
+    if keys:
+        outs.append(np.concatenate(out_key, 0) if out_key else np.zeros((0, 3), np.int64))
+    return tuple(outs) if len(outs) > 1 else tris
 
 
 def _less(a, b):

@@ -145,7 +145,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_track_normal_eq.argtypes = [vp, C.c_int32, fp, P(TsdfView), vp, C.c_float, vp, vp, vp]
     L.tsdf_se3_gauss_newton_step.argtypes = [dp, dp, dp, dp, dp]
     L.tsdf_clean.argtypes = [vp, P(C.c_uint64), vp]
-    L.tsdf_extract_mesh.argtypes = [vp, vp, vp, C.c_uint64, P(C.c_uint64), vp]
+    L.tsdf_extract_mesh.argtypes = [vp, vp, vp, vp, C.c_uint64, P(C.c_uint64), vp]
     L.tsdf_icp_default_params.argtypes = [P(TsdfIcpParams)]
     L.tsdf_icp_default_params.restype = None
     L.tsdf_depth_to_points.argtypes = [vp, P(TsdfView), vp, vp]

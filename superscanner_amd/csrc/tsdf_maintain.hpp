@@ -195,13 +195,27 @@ __device__ __forceinline__ int tet_triangles(const float (&p)[8][3], const float
     return nt;
 }
 
+// Key of the vertex on the edge from voxel (gx, gy, gz) (inside) to its
+// neighbour at offset (dx, dy, dz) (outside): the voxel's global index,
+// biased by 2^18 per axis (19 bits each), and the offset (5 bits).  Equal
+// keys are equal vertices bit for bit.
+constexpr int kKeyBias = 1 << 18;
+
+__device__ __forceinline__ unsigned long long vertex_key(int gx, int gy, int gz, int dx, int dy, int dz)
+{
+    return ((unsigned long long)(gx + kKeyBias) << 43) | ((unsigned long long)(gy + kKeyBias) << 24) |
+           ((unsigned long long)(gz + kKeyBias) << 5) | (unsigned long long)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
+}
+
 // One workgroup per brick, two cells per thread.  tris: [cap][3][3] world
 // coordinates; cols (kColor): [cap][3][3] vertex colours interpolated like
-// the positions; count: total triangles (may exceed cap).  A brick that was
-// allocated but never written (coords.w bit 0) counts as unobserved.
+// the positions; keys (or null): [cap][3] vertex keys; count: total
+// triangles (may exceed cap).  A brick that was allocated but never written
+// (coords.w bit 0) counts as unobserved.
 template <typename T, bool kColor>
 __global__ __launch_bounds__(kThreads) void k_mesh(GridView g, unsigned int n, float* tris, float* cols,
-                                                   unsigned long long cap, unsigned long long* count)
+                                                   unsigned long long* keys, unsigned long long cap,
+                                                   unsigned long long* count)
 {
     __shared__ int nb[8];  // brick ids of (dx, dy, dz) = (c & 1, c >> 1 & 1, c >> 2 & 1)
     __shared__ int4 bc;
@@ -268,6 +282,11 @@ __global__ __launch_bounds__(kThreads) void k_mesh(GridView g, unsigned int n, f
                     edge_point(p, s, a, bb, v);
                     for (int r = 0; r < 3; ++r)
                         tris[k * 9 + q * 3 + r] = v[r];
+                    if (keys) {
+                        const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+                        keys[k * 3 + q] = vertex_key(bc.x * 8 + x + ax, bc.y * 8 + y + ay, bc.z * 8 + z + az,
+                                                     (bb & 1) - ax, ((bb >> 1) & 1) - ay, ((bb >> 2) & 1) - az);
+                    }
                     if (kColor) {
                         const float u = s[a] / (s[a] - s[bb]);
                         const float* ca = g.rgb + (size_t)vbrick[a] * 3 * kVPB + vid[a];
@@ -375,8 +394,8 @@ tsdf_status tsdf_clean(tsdf_grid* g, uint64_t* n_removed, void* stream)
     return out;
 }
 
-tsdf_status tsdf_extract_mesh(tsdf_grid* g, float* triangles, float* colors, uint64_t cap, uint64_t* n_triangles,
-                              void* stream)
+tsdf_status tsdf_extract_mesh(tsdf_grid* g, float* triangles, float* colors, uint64_t* vertex_keys, uint64_t cap,
+                              uint64_t* n_triangles, void* stream)
 {
     if (!g || !n_triangles || (cap && !triangles))
         return fail(TSDF_BAD_ARG, "NULL grid, count or triangle buffer");
@@ -390,27 +409,35 @@ tsdf_status tsdf_extract_mesh(tsdf_grid* g, float* triangles, float* colors, uin
     const unsigned int n = (unsigned int)st.n_bricks;
     if (n == 0)
         return TSDF_OK;
+    if (vertex_keys) {
+        // every block ever inserted lies in the control block's bounding box
+        const Ctrl& c = *g->ctrl_host;
+        for (int a = 0; a < 3; ++a)
+            if (c.bbox_min[a] < -(kKeyBias / 8) || c.bbox_max[a] >= kKeyBias / 8 - 1)
+                return fail(TSDF_BAD_ARG, "map extent beyond the vertex key range (+-32767 blocks per axis)");
+    }
     DeviceScope ds(g->p.device);
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* cnt = nullptr;
     HIPC(hipMalloc(&cnt, sizeof(unsigned long long)));
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, cnt, (size_t)1);
+    unsigned long long* keys = cap > 0 ? (unsigned long long*)vertex_keys : nullptr;
     const int grid = (int)(n < 65536u ? n : 65536u);
     const bool f16 = g->p.storage == TSDF_STORAGE_F16;
     const bool col = colors != nullptr && cap > 0;
     const unsigned long long c = (unsigned long long)cap;
     if (f16 && col)
-        hipLaunchKernelGGL((k_mesh<__half, true>), dim3(grid), dim3(kThreads), 0, s, view(g), n, triangles, colors, c,
-                           cnt);
+        hipLaunchKernelGGL((k_mesh<__half, true>), dim3(grid), dim3(kThreads), 0, s, view(g), n, triangles, colors, keys,
+                           c, cnt);
     else if (f16)
         hipLaunchKernelGGL((k_mesh<__half, false>), dim3(grid), dim3(kThreads), 0, s, view(g), n, triangles, colors,
-                           c, cnt);
+                           keys, c, cnt);
     else if (col)
-        hipLaunchKernelGGL((k_mesh<float, true>), dim3(grid), dim3(kThreads), 0, s, view(g), n, triangles, colors, c,
-                           cnt);
+        hipLaunchKernelGGL((k_mesh<float, true>), dim3(grid), dim3(kThreads), 0, s, view(g), n, triangles, colors, keys,
+                           c, cnt);
     else
         hipLaunchKernelGGL((k_mesh<float, false>), dim3(grid), dim3(kThreads), 0, s, view(g), n, triangles, colors,
-                           c, cnt);
+                           keys, c, cnt);
     unsigned long long h = 0;
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)

@@ -17,6 +17,7 @@ libtsdf_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+from contextlib import nullcontext as _nullcontext
 
 import numpy as np
 import torch
@@ -270,41 +271,49 @@ class GPUVoxelGrid:
 
     def extract_mesh(self, stream=None) -> dict:
         """Mesh of the zero level set (stands in for marchingCubesHost,
-        GPUVoxelGrid.cpp:200-251, and the ComputeVertexNormals that follows):
-        {"triangles": [m, 3, 3] device float32 soup, "triangle_colors":
-        [m, 3, 3] device float32 or None, "vertices": [k, 3] host float32
-        merged by exact coordinates, "faces": [m, 3] host int64,
-        "normals": [k, 3] host float32 unit area-weighted vertex normals,
-        "colors": [k, 3] host float32 or None}."""
+        GPUVoxelGrid.cpp:200-251, and the ComputeVertexNormals that follows),
+        welded on the GPU by vertex key: {"triangles": [m, 3, 3] device
+        float32 soup, "triangle_colors": [m, 3, 3] device float32 or None,
+        "vertices": [k, 3] float32, "faces": [m, 3] int64, "normals": [k, 3]
+        float32 unit area-weighted vertex normals (zero where every face of
+        the vertex has zero area), "colors": [k, 3] float32 or None} -- the
+        last four as host numpy arrays."""
         cap = max(1024, 64 * self.size())
         while True:
             tris = torch.empty((cap, 3, 3), dtype=torch.float32, device=self.device)
             cols = torch.empty((cap, 3, 3), dtype=torch.float32, device=self.device) if self.withColor else None
+            keys = torch.empty((cap, 3), dtype=torch.int64, device=self.device)
             n = C.c_uint64(0)
             check(self._L.tsdf_extract_mesh(self._h, C.c_void_p(tris.data_ptr()),
-                                            C.c_void_p(cols.data_ptr()) if cols is not None else None, cap,
-                                            C.byref(n), _stream_ptr(stream)), "tsdf_extract_mesh")
+                                            C.c_void_p(cols.data_ptr()) if cols is not None else None,
+                                            C.c_void_p(keys.data_ptr()), cap, C.byref(n), _stream_ptr(stream)),
+                  "tsdf_extract_mesh")
             if n.value <= cap:
                 break
             cap = int(n.value)
         m = int(n.value)
-        tris = tris[:m]
-        cols = cols[:m] if cols is not None else None
-        soup = tris.reshape(-1, 3).cpu().numpy()
-        verts, first, inv = np.unique(soup.view(np.uint32).reshape(-1, 3), axis=0, return_index=True,
-                                      return_inverse=True)
-        verts = verts.view(np.float32).reshape(-1, 3)
-        faces = inv.reshape(-1, 3).astype(np.int64)
-        t = soup.reshape(-1, 3, 3).astype(np.float64)
-        fn = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
-        vn = np.zeros((len(verts), 3))
-        for q in range(3):
-            np.add.at(vn, faces[:, q], fn)
-        norm = np.linalg.norm(vn, axis=1, keepdims=True)
-        vn = (vn / np.where(norm > 0, norm, 1.0)).astype(np.float32)
-        vcol = cols.reshape(-1, 3).cpu().numpy()[first] if cols is not None else None
-        return {"triangles": tris, "triangle_colors": cols, "vertices": verts, "faces": faces, "normals": vn,
-                "colors": vcol}
+        with torch.cuda.stream(stream) if stream is not None else _nullcontext():
+            tris = tris[:m]
+            cols = cols[:m] if cols is not None else None
+            uk, inv = torch.unique(keys[:m].reshape(-1), return_inverse=True)
+            k = uk.numel()
+            verts = torch.zeros((k, 3), dtype=torch.float32, device=self.device)
+            verts.index_copy_(0, inv, tris.reshape(-1, 3))  # equal keys carry equal positions
+            faces = inv.reshape(-1, 3)
+            t = tris.double()
+            fn = torch.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0], dim=1)
+            vn = torch.zeros((k, 3), dtype=torch.float64, device=self.device)
+            for q in range(3):
+                vn.index_add_(0, faces[:, q], fn)
+            norm = vn.norm(dim=1, keepdim=True)
+            vn = torch.where(norm > 0, vn / torch.where(norm > 0, norm, torch.ones_like(norm)), vn).float()
+            vcol = None
+            if cols is not None:
+                vcol = torch.zeros((k, 3), dtype=torch.float32, device=self.device)
+                vcol.index_copy_(0, inv, cols.reshape(-1, 3))
+        return {"triangles": tris, "triangle_colors": cols, "vertices": verts.cpu().numpy(),
+                "faces": faces.cpu().numpy(), "normals": vn.cpu().numpy(),
+                "colors": vcol.cpu().numpy() if vcol is not None else None}
 
     def sdf_tensor(self) -> torch.Tensor:
         """The stored sdf values as a device float32 tensor [n_bricks, 512]

@@ -47,20 +47,39 @@ def oracle_mesh_of(g):
     return mesh.extract_mesh(ex["coords"], ex["sdf"], ex["w"], g.vl)
 
 
+def mesh_with_keys(g):
+    m = g.extract_mesh()
+    n = m["triangles"].shape[0]
+    keys = torch.empty((max(n, 1), 3), dtype=torch.int64, device=g.device)
+    tris = torch.empty((max(n, 1), 3, 3), dtype=torch.float32, device=g.device)
+    cnt = C.c_uint64(0)
+    assert g._L.tsdf_extract_mesh(g._h, C.c_void_p(tris.data_ptr()), None, C.c_void_p(keys.data_ptr()), n,
+                                  C.byref(cnt), None) == 0 and cnt.value == n
+    return m, tris[:n].cpu().numpy(), keys[:n].cpu().numpy()
+
+
+def key_map(tris, keys):
+    return dict(zip(keys.reshape(-1).tolist(), map(tuple, tris.reshape(-1, 3).view(np.uint32).tolist())))
+
+
 def assert_mesh_equal(g):
-    got = g.extract_mesh()
-    exp = oracle_mesh_of(g)
+    got, tk, kg = mesh_with_keys(g)
+    ex = {k: v.cpu().numpy() for k, v in g.export_bricks().items()}
+    exp, ko = mesh.extract_mesh(ex["coords"], ex["sdf"], ex["w"], g.vl, keys=True)
     tg = got["triangles"].cpu().numpy()
     assert tg.shape == exp.shape, (tg.shape, exp.shape)
     assert np.array_equal(mesh.canonical(tg), mesh.canonical(exp)), "triangle multisets differ"
-    # merged vertices index the same soup
+    # the same vertex keys name the same positions
+    assert key_map(tk, kg) == key_map(exp, ko), "vertex keys differ"
+    # the welded mesh indexes the same soup, one vertex per key
     assert_bits_equal(got["vertices"][got["faces"]], tg, "indexed mesh")
+    assert got["vertices"].shape[0] == len(np.unique(ko))
     return tg
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_mesh_equals_oracle(gpu, storage):
-    cfg, intr, poses, depth = small_frames(n=10 if storage == "f32" else 4)
+    cfg, intr, poses, depth = small_frames(n=4 if storage == "f32" else 2)
     g = GPUVoxelGrid(1 << 26, cfg["truncation"], cfg["voxel_length"], 240, 320, n_buckets=1 << 18,
                      n_bricks=1 << 17, storage=storage)
     g.setIntrinsic(*intr)
@@ -114,17 +133,17 @@ def test_mesh_short_buffer(gpu):
     cap = 100
     buf = torch.full((cap + 1, 3, 3), 7.0, dtype=torch.float32, device=gpu)
     n = C.c_uint64(0)
-    assert g._L.tsdf_extract_mesh(g._h, C.c_void_p(buf.data_ptr()), None, cap, C.byref(n), None) == 0
+    assert g._L.tsdf_extract_mesh(g._h, C.c_void_p(buf.data_ptr()), None, None, cap, C.byref(n), None) == 0
     assert n.value == len(full) > cap
     part = buf.cpu().numpy()
     assert (part[cap] == 7.0).all()  # nothing written past the cap
     fc = {tuple(r) for r in mesh.canonical(full).tolist()}
     assert all(tuple(r) in fc for r in mesh.canonical(part[:cap]).tolist())
     # a zero-capacity call only counts
-    assert g._L.tsdf_extract_mesh(g._h, None, None, 0, C.byref(n), None) == 0
+    assert g._L.tsdf_extract_mesh(g._h, None, None, None, 0, C.byref(n), None) == 0
     assert n.value == len(full)
     # colours from a grid without colour are refused before any launch
-    assert g._L.tsdf_extract_mesh(g._h, C.c_void_p(buf.data_ptr()), C.c_void_p(buf.data_ptr()), cap, C.byref(n),
+    assert g._L.tsdf_extract_mesh(g._h, C.c_void_p(buf.data_ptr()), C.c_void_p(buf.data_ptr()), None, cap, C.byref(n),
                                   None) == _lib.TSDF_BAD_ARG
     g.close()
 

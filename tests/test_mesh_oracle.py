@@ -226,6 +226,43 @@ def test_ply_round_trip(tmp_path):
     assert np.array_equal(r["vertices"][r["faces"]], t)  # the indexed mesh is the soup
 
 
+def decode_key(k):
+    k = np.asarray(k, np.int64)
+    m19 = (1 << 19) - 1
+    g = np.stack([(k >> 43) & m19, (k >> 24) & m19, (k >> 5) & m19], -1) - (1 << 18)
+    c = k & 31
+    d = np.stack([c % 3, (c // 3) % 3, c // 9], -1) - 1
+    return g, d
+
+
+def test_vertex_keys_name_their_edges():
+    coords, sdf, w = sphere()
+    t, keys = mesh.extract_mesh(coords, sdf, w, VL, keys=True)
+    assert np.array_equal(t, mesh.extract_mesh(coords, sdf, w, VL))
+    flat_k = keys.reshape(-1)
+    flat_p = t.reshape(-1, 3).view(np.uint32)
+    uk, first, inv = np.unique(flat_k, return_index=True, return_inverse=True)
+    # a key names one position, and distinct keys distinct positions here
+    assert np.array_equal(flat_p, flat_p[first][inv])
+    assert len(np.unique(flat_p, axis=0)) == len(uk)
+    # the vertex lies on the segment between the two voxel centres the key names
+    g, d = decode_key(flat_k)
+    pa = (g + 0.5) * VL
+    pb = (g + d + 0.5) * VL
+    v = t.reshape(-1, 3).astype(np.float64)
+    u = ((v - pa) * (pb - pa)).sum(1) / ((pb - pa) ** 2).sum(1)
+    assert (u >= -1e-6).all() and (u <= 1 + 1e-6).all()
+    assert np.abs(pa + u[:, None] * (pb - pa) - v).max() < 1e-6
+    assert (np.abs(d).sum(1) > 0).all()
+    # welded by key, the sphere is closed and consistently oriented
+    f = inv.reshape(-1, 3)
+    de = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    n = len(uk)
+    kk = de[:, 0].astype(np.int64) * n + de[:, 1]
+    ukk, cnt = np.unique(kk, return_counts=True)
+    assert (cnt == 1).all() and np.isin(de[:, 1].astype(np.int64) * n + de[:, 0], ukk).all()
+
+
 @pytest.mark.parametrize("m", [1, 2, 3])
 def test_planted_mistakes_are_caught(m):
     mesh.set_mutation(m)
