@@ -40,6 +40,32 @@ def write_ply(path, mesh: dict) -> None:
         fh.write(frec.tobytes())
 
 
+FLIP = np.diag([1.0, -1.0, -1.0, 1.0])  # getflip, matrixutil.cpp:27-35
+
+
+def save_cpu_mesh(path, mesh: dict, flip: bool = True) -> None:
+    """Model::saveCPUMesh, Model.cpp:205-228: a copy of the mesh whose colour
+    channels outside the valid range are set to 0 (the reference tests its
+    [0, 1] Open3D colours; these are 0..255), transformed by getflip
+    (vertices and normals; a rotation, so the winding is kept), written as
+    binary PLY with normals and colours."""
+    m = dict(mesh)
+    v = np.asarray(m["vertices"], np.float64).reshape(-1, 3)
+    nrm = m.get("normals")
+    if flip:
+        v = v @ FLIP[:3, :3].T
+        if nrm is not None:
+            nrm = np.asarray(nrm, np.float64).reshape(-1, 3) @ FLIP[:3, :3].T
+    col = m.get("colors")
+    if col is not None:
+        col = np.asarray(col, np.float64).reshape(-1, 3).copy()
+        col[(col > 255.0) | (col < 0.0)] = 0.0
+    m["vertices"] = v.astype(np.float32)
+    m["normals"] = nrm.astype(np.float32) if nrm is not None else None
+    m["colors"] = col
+    write_ply(path, m)
+
+
 def read_ply(path) -> dict:
     """Reads what write_ply wrote (binary little-endian, the same properties)."""
     with open(path, "rb") as fh:

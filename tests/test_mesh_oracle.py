@@ -224,6 +224,12 @@ def test_ply_round_trip(tmp_path):
     assert np.array_equal(r["normals"], m["normals"])
     assert (r["colors"] == [255, 1, 255]).all()
     assert np.array_equal(r["vertices"][r["faces"]], t)  # the indexed mesh is the soup
+    # saveCPUMesh: out-of-range channels zeroed, y and z flipped
+    mesh_io.save_cpu_mesh(tmp_path / "s.ply", m)
+    r = mesh_io.read_ply(tmp_path / "s.ply")
+    assert (r["colors"] == [0, 1, 0]).all()  # 255.4 and 300 are out of range
+    assert np.array_equal(r["vertices"], m["vertices"] * np.float32([1, -1, -1]))
+    assert np.array_equal(r["normals"], np.tile(np.float32([0, 0, -1]), (len(verts), 1)))
 
 
 def decode_key(k):
