@@ -147,8 +147,8 @@ __device__ int find_or_insert(const GridView& g, uint32_t key, int x, int y, int
                     // every inserted key counts for the raycast's empty-space
                     // certificate, whether or not its brick was allocated
                     const int sb = sb_index(x, y, z);
-                    if (sb >= 0)
-                        atomicOr(&g.sb_bits[sb >> 5], 1u << (sb & 31));
+                    if (sb >= 0 && !(atomicOr(&g.sb_bits[sb >> 5], 1u << (sb & 31)) & (1u << (sb & 31))))
+                        atomicAdd(&g.ctrl->sb_version, 1u);  // a new superblock: the distance field is stale
                     atomicMin(&g.ctrl->bbox_min[0], x);
                     atomicMin(&g.ctrl->bbox_min[1], y);
                     atomicMin(&g.ctrl->bbox_min[2], z);
@@ -264,6 +264,7 @@ __global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
             z.bbox_min[a] = 0x7FFFFFFF;
             z.bbox_max[a] = (int)0x80000000u;
         }
+        z.sb_version = g.ctrl->sb_version + 1u;  // the bits were cleared
         *g.ctrl = z;
     }
 }
@@ -785,23 +786,144 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
 }
 
 // ---------------------------------------------------------------------------
-// k_raycast: raycastingTag, GPUVoxelGrid.cpp:561-715 (march in tsdf_march.hpp).
-// One pixel per lane, 8 x 8 pixels per wavefront, 16 x 16 per workgroup.
+// k_sbdf: the raycast's empty-space distance over superblocks (SbdfInfo),
+// rebuilt only when Ctrl::sb_version moved since the last build.  One
+// workgroup: the box of set superblocks is small (a 5 x 4 m room at 4 mm
+// voxels is ~3.5 k superblocks); the passes are those of sbdf_build_host.
 // ---------------------------------------------------------------------------
-template <typename T, bool kColor, bool kStats>
-__global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, RayParams rp, float* points,
-                                                      float* colors, float* depth, uint8_t* ub, Footprint fp)
+constexpr int kSbdfThreads = 1024;
+
+__global__ __launch_bounds__(kSbdfThreads) void k_sbdf(GridView g, uint8_t* tmp)
 {
-    const int tiles_x = (rp.W + 15) / 16;
-    const int n_tiles = tiles_x * ((rp.H + 15) / 16);
+    __shared__ int sh[8];  // lo[3], dim[3], valid, skip
+    if (threadIdx.x == 0) {
+        const unsigned int ver = g.ctrl->sb_version;
+        const SbdfInfo inf = *g.sbdf_info;
+        int valid = 1;
+        int lo[3] = {0, 0, 0}, dim[3] = {0, 0, 0};
+        if (g.ctrl->bbox_min[0] <= g.ctrl->bbox_max[0]) {
+            size_t vol = 1;
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = g.ctrl->bbox_min[a] >> 3;
+                const int hi = g.ctrl->bbox_max[a] >> 3;
+                dim[a] = hi - lo[a] + 1;
+                valid = valid && lo[a] + kSbHalf >= 0 && hi + kSbHalf < kSbSide;
+                vol *= (size_t)dim[a];
+            }
+            valid = valid && vol <= kSbdfCap;
+        }
+        for (int a = 0; a < 3; ++a) {
+            sh[a] = lo[a];
+            sh[3 + a] = dim[a];
+        }
+        sh[6] = valid;
+        sh[7] = inf.valid && inf.version == ver;
+        if (!sh[7] && (!valid || dim[0] == 0)) {
+            SbdfInfo o;
+            o.version = ver;
+            o.valid = valid;
+            for (int a = 0; a < 3; ++a) {
+                o.lo[a] = lo[a];
+                o.dim[a] = dim[a];
+            }
+            *g.sbdf_info = o;
+            sh[7] = 1;  // nothing to compute
+        }
+    }
+    __syncthreads();
+    if (sh[7])
+        return;
+    const int lo[3] = {sh[0], sh[1], sh[2]};
+    const int nx = sh[3], ny = sh[4], nz = sh[5];
+    const int vol = nx * ny * nz;
+    uint8_t* out = g.sbdf;
+    // x: distance to the nearest set bit of the row, both directions
+    for (int r = threadIdx.x; r < ny * nz; r += kSbdfThreads) {
+        const int y = r % ny, z = r / ny;
+        uint8_t* row = out + (size_t)nx * r;
+        int d = kSbdfMax;
+        for (int x = 0; x < nx; ++x) {
+            d = sb_bit(g.sb_bits, x + lo[0], y + lo[1], z + lo[2]) ? 0 : (d < kSbdfMax ? d + 1 : kSbdfMax);
+            row[x] = (uint8_t)d;
+        }
+        d = kSbdfMax;
+        for (int x = nx - 1; x >= 0; --x) {
+            const int v = row[x];
+            d = v == 0 ? 0 : (d < kSbdfMax ? d + 1 : kSbdfMax);
+            row[x] = (uint8_t)(v < d ? v : d);
+        }
+    }
+    __syncthreads();
+    // y, then z: min over the window of max(|offset|, value)
+    for (int c = threadIdx.x; c < vol; c += kSbdfThreads) {
+        const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+        int best = kSbdfMax;
+        const int y0 = y - kSbdfMax > 0 ? y - kSbdfMax : 0, y1 = y + kSbdfMax < ny - 1 ? y + kSbdfMax : ny - 1;
+        for (int yy = y0; yy <= y1; ++yy) {
+            const int a = yy > y ? yy - y : y - yy;
+            const int v = out[x + nx * (yy + ny * z)];
+            const int m = a > v ? a : v;
+            best = m < best ? m : best;
+        }
+        tmp[c] = (uint8_t)best;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < vol; c += kSbdfThreads) {
+        const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+        int best = kSbdfMax;
+        const int z0 = z - kSbdfMax > 0 ? z - kSbdfMax : 0, z1 = z + kSbdfMax < nz - 1 ? z + kSbdfMax : nz - 1;
+        for (int zz = z0; zz <= z1; ++zz) {
+            const int a = zz > z ? zz - z : z - zz;
+            const int v = tmp[x + nx * (y + ny * zz)];
+            const int m = a > v ? a : v;
+            best = m < best ? m : best;
+        }
+        out[c] = (uint8_t)best;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        SbdfInfo o;
+        o.version = g.ctrl->sb_version;
+        o.valid = 1;
+        for (int a = 0; a < 3; ++a) {
+            o.lo[a] = lo[a];
+            o.dim[a] = sh[3 + a];
+        }
+        *g.sbdf_info = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_raycast: raycastingTag, GPUVoxelGrid.cpp:561-715 (march in tsdf_march.hpp).
+// One pixel per lane, 8 x 8 pixels per wavefront; a workgroup of kRayWaves
+// wavefronts takes a (8 kRayWx) x (8 kRayWy) tile.
+// ---------------------------------------------------------------------------
+#ifndef TSDF_RAY_WAVES
+#define TSDF_RAY_WAVES 4
+#endif
+constexpr int kRayWaves = TSDF_RAY_WAVES;  // 1, 2 or 4
+constexpr int kRayWx = kRayWaves >= 2 ? 2 : 1, kRayWy = kRayWaves >= 4 ? 2 : 1;
+constexpr int kRayTileW = 8 * kRayWx, kRayTileH = 8 * kRayWy;
+
+template <typename T, bool kColor, bool kStats>
+__global__ __launch_bounds__(64 * kRayWaves)
+#ifdef TSDF_RAY_WAVES_PER_EU
+__attribute__((amdgpu_waves_per_eu(TSDF_RAY_WAVES_PER_EU, TSDF_RAY_WAVES_PER_EU)))
+#endif
+void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors, float* depth, uint8_t* ub,
+               Footprint fp)
+{
+    const int tiles_x = (rp.W + kRayTileW - 1) / kRayTileW;
+    const int n_tiles = tiles_x * ((rp.H + kRayTileH - 1) / kRayTileH);
     // Workgroups are dealt to the 8 XCDs round-robin; with xcd_map the
     // workgroups of one XCD take one contiguous band of tiles, so the bricks
     // one XCD's L2 holds belong to neighbouring rays.
     const int per = (n_tiles + 7) >> 3;
     const int tile = rp.xcd_map ? (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = (tile / tiles_x) * 16 + (wv >> 1) * 8 + (lane >> 3);
-    const int j = (tile % tiles_x) * 16 + (wv & 1) * 8 + (lane & 7);
+    const int i = (tile / tiles_x) * kRayTileH + (wv / kRayWx) * 8 + (lane >> 3);
+    const int j = (tile % tiles_x) * kRayTileW + (wv % kRayWx) * 8 + (lane & 7);
     // empty-space certificate for this raycast (identical in every lane):
     // probed blocks lie within `reach` blocks of the camera's block
     RayParams rq = rp;
@@ -819,6 +941,17 @@ __global__ __launch_bounds__(kThreads) void k_raycast(GridView g, Pose pose, Ray
             bmax[a] = g.ctrl->bbox_max[a];
         }
         rq.use_sb = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
+        // and a distance field of the current superblock bits
+        const SbdfInfo inf = *g.sbdf_info;
+        if (rq.use_sb && inf.valid && inf.version == g.ctrl->sb_version) {
+            rq.df = g.sbdf;
+            for (int a = 0; a < 3; ++a) {
+                rq.df_lo[a] = inf.lo[a];
+                rq.df_dim[a] = inf.dim[a];
+            }
+        } else {
+            rq.use_sb = 0;
+        }
     }
     unsigned int hit = 0, bad = 0, samples = 0, probes = 0;
     if (tile < n_tiles && i < rp.H && j < rp.W) {
@@ -915,6 +1048,11 @@ struct tsdf_grid {
     float* rgb;
     uint32_t* touched;
     uint32_t* sb_bits;
+    uint64_t map_gen;    // bumped by every call that can set or clear superblock bits
+    uint64_t sbdf_gen;   // map_gen at the last distance-field refresh
+    uint8_t* sbdf;       // empty-space distance over superblocks, and its scratch
+    uint8_t* sbdf_tmp;
+    SbdfInfo* sbdf_info;
     float4* ptmap;
     Ctrl* ctrl;
     Ctrl* ctrl_host;  // pinned
@@ -996,6 +1134,8 @@ GridView view(const tsdf_grid* g)
     v.rgb = g->rgb;
     v.touched = g->touched;
     v.sb_bits = g->sb_bits;
+    v.sbdf = g->sbdf;
+    v.sbdf_info = g->sbdf_info;
     v.ctrl = g->ctrl;
     v.mask = g->p.n_buckets - 1u;
     v.n_bricks = g->p.n_bricks;
@@ -1183,29 +1323,41 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
     rp.W = g->p.width;
     rp.xcd_map = g->tune.raycast_xcd_map;
     rp.use_sb = g->tune.raycast_superblocks;
-    const int nt = ((rp.W + 15) / 16) * ((rp.H + 15) / 16);
+    rp.df = nullptr;
+    for (int a = 0; a < 3; ++a)
+        rp.df_lo[a] = rp.df_dim[a] = 0;
+    if (rp.use_sb && !kStats && g->map_gen != g->sbdf_gen) {
+        // a call since the last check may have set superblock bits: refresh
+        // the distance field if they moved (the kernel compares versions)
+        hipLaunchKernelGGL(k_sbdf, dim3(1), dim3(kSbdfThreads), 0, s, view(g), g->sbdf_tmp);
+        g->sbdf_gen = g->map_gen;
+    }
+    const int nt = ((rp.W + kRayTileW - 1) / kRayTileW) * ((rp.H + kRayTileH - 1) / kRayTileH);
     const dim3 grid(((nt + 7) / 8) * 8);
+    const dim3 block(64 * kRayWaves);
     const bool color = g->p.with_color != 0;
     if (g->p.storage == TSDF_STORAGE_F16) {
         if (color)
-            hipLaunchKernelGGL((k_raycast<__half, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp, points,
+            hipLaunchKernelGGL((k_raycast<__half, true, kStats>), grid, block, 0, s, view(g), P, rp, points,
                                colors, depth, ub, fp);
         else
-            hipLaunchKernelGGL((k_raycast<__half, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp,
+            hipLaunchKernelGGL((k_raycast<__half, false, kStats>), grid, block, 0, s, view(g), P, rp,
                                points, colors, depth, ub, fp);
     } else {
         if (color)
-            hipLaunchKernelGGL((k_raycast<float, true, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp, points,
+            hipLaunchKernelGGL((k_raycast<float, true, kStats>), grid, block, 0, s, view(g), P, rp, points,
                                colors, depth, ub, fp);
         else
-            hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, dim3(kThreads), 0, s, view(g), P, rp, points,
+            hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, block, 0, s, view(g), P, rp, points,
                                colors, depth, ub, fp);
     }
 }
 
 }  // namespace
 
+
 extern "C" {
+
 
 void tsdf_default_params(tsdf_params* p)
 {
@@ -1325,8 +1477,19 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
         alloc((void**)&g->rgb, nbr * 3 * kVPB * sizeof(float));
     alloc((void**)&g->touched, nb * sizeof(uint32_t));
     alloc((void**)&g->sb_bits, kSbWords * sizeof(uint32_t));
+    alloc((void**)&g->sbdf, kSbdfCap);
+    alloc((void**)&g->sbdf_tmp, kSbdfCap);
+    alloc((void**)&g->sbdf_info, sizeof(SbdfInfo));
     alloc((void**)&g->ptmap, (size_t)p->height * p->width * sizeof(float4));
     alloc((void**)&g->ctrl, sizeof(Ctrl));
+    if (st == TSDF_OK) {
+        // no distance field yet; counters start from zero
+        hipError_t e = hipMemset(g->sbdf_info, 0, sizeof(SbdfInfo));
+        if (e == hipSuccess)
+            e = hipMemset(g->ctrl, 0, sizeof(Ctrl));
+        if (e != hipSuccess)
+            st = fail(TSDF_HIP_ERROR, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
     alloc((void**)&g->view_dev, 256 * sizeof(ViewPose));
     if (st == TSDF_OK) {
         hipError_t e = hipHostMalloc((void**)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
@@ -1364,6 +1527,9 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->rgb);
     (void)hipFree(g->touched);
     (void)hipFree(g->sb_bits);
+    (void)hipFree(g->sbdf);
+    (void)hipFree(g->sbdf_tmp);
+    (void)hipFree(g->sbdf_info);
     (void)hipFree(g->ptmap);
     (void)hipFree(g->ctrl);
     (void)hipFree(g->op_mask);
@@ -1408,6 +1574,7 @@ tsdf_status tsdf_reset(tsdf_grid* g, void* stream)
     }
     HIPC(hipGetLastError());
     g->epoch = 0;
+    ++g->map_gen;
     return TSDF_OK;
 }
 
@@ -1450,6 +1617,7 @@ tsdf_status tsdf_integrate(tsdf_grid* g, const uint16_t* depth, int32_t pitch, c
     op.sign = sign;
     make_pose(E, &op.pose);
     const uint32_t epoch = ++g->epoch;
+    ++g->map_gen;
     const int par = (int)(epoch & 1u);
     const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH);
     {
@@ -1580,6 +1748,7 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
         const int passes = (sn + per - 1) / per;
         st->epoch_base = g->epoch + 1;
         g->epoch += (uint32_t)passes;
+        ++g->map_gen;
         // one captured graph (staging copy + all passes) when the shape
         // allows; per-kernel profiling needs the separate launches
         if (g->tune.batch_graph && !g->prof_on && n_ops <= kMaxOps) {

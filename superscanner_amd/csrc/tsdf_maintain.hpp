@@ -74,11 +74,13 @@ __global__ __launch_bounds__(kThreads) void k_rehash_clear(GridView g)
     }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < kSbWords; i += stride)
         g.sb_bits[i] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int a = 0; a < 3; ++a) {
             g.ctrl->bbox_min[a] = 0x7FFFFFFF;
             g.ctrl->bbox_max[a] = (int)0x80000000u;
         }
+        g.ctrl->sb_version += 1u;  // the bits were cleared
+    }
 }
 
 // inserts brick j (already at its final id) under its reference key
@@ -95,8 +97,8 @@ __global__ __launch_bounds__(kThreads) void k_rehash_insert(GridView g, unsigned
             b = (b + 1) & g.mask;
         }
         const int sb = sb_index(c.x, c.y, c.z);
-        if (sb >= 0)
-            atomicOr(&g.sb_bits[sb >> 5], 1u << (sb & 31));
+        if (sb >= 0 && !(atomicOr(&g.sb_bits[sb >> 5], 1u << (sb & 31)) & (1u << (sb & 31))))
+            atomicAdd(&g.ctrl->sb_version, 1u);
         atomicMin(&g.ctrl->bbox_min[0], c.x);
         atomicMin(&g.ctrl->bbox_min[1], c.y);
         atomicMin(&g.ctrl->bbox_min[2], c.z);
@@ -321,6 +323,7 @@ tsdf_status tsdf_clean(tsdf_grid* g, uint64_t* n_removed, void* stream)
         return TSDF_OK;
     DeviceScope ds(g->p.device);
     hipStream_t s = (hipStream_t)stream;
+    ++g->map_gen;  // the table and superblock bits are rebuilt
     const bool f16 = g->p.storage == TSDF_STORAGE_F16;
     const size_t tsz = f16 ? sizeof(__half) : sizeof(float);
     uint8_t* alive_d = nullptr;

@@ -31,6 +31,21 @@ struct Ctrl {
     unsigned long long ray_probes;
     int bbox_min[3];  // bounding box of every block ever inserted since reset
     int bbox_max[3];
+    unsigned int sb_version;  // bumped when a superblock bit is set or the bits are cleared
+    unsigned int pad_;
+};
+
+// Empty-space distance over superblocks: for the superblocks inside the box
+// of the set superblock bits, the Chebyshev distance (in superblocks, capped
+// at kSbdfMax) to the nearest set bit.  Rebuilt by k_sbdf whenever
+// Ctrl::sb_version moved; `version` is the Ctrl::sb_version it was built from.
+constexpr int kSbdfMax = 31;
+constexpr size_t kSbdfCap = (size_t)1 << 21;  // cells of the box
+struct SbdfInfo {
+    unsigned int version;
+    int valid;     // 0: not usable (box outside the bitmap or larger than kSbdfCap)
+    int lo[3];     // superblock coordinates of the box origin
+    int dim[3];    // box size in superblocks (0: nothing inserted)
 };
 
 // Device view of one grid (all pointers are device memory on the GPU).
@@ -42,6 +57,8 @@ struct GridView {
     float* rgb;                 // [n_bricks][3][512] or null
     uint32_t* touched;          // [n_buckets] slots touched by the current call
     uint32_t* sb_bits;          // superblock occupancy, kSbSide^3 bits
+    uint8_t* sbdf;              // [kSbdfCap] empty-space distance over superblocks
+    SbdfInfo* sbdf_info;
     Ctrl* ctrl;
     uint32_t mask;              // n_buckets - 1
     uint32_t n_bricks;
@@ -220,7 +237,9 @@ struct RayParams {
     float back;       // vbl - d_trunc, GPUVoxelGrid.cpp:613
     int H, W;
     int xcd_map;      // 1: each XCD takes one band of the image
-    int use_sb;       // superblock shortcut allowed (alias certificate holds)
+    int use_sb;       // empty-space replay allowed (alias certificate and distance field hold)
+    const uint8_t* df;  // the distance field and its box (filled in by the kernel)
+    int df_lo[3], df_dim[3];
 };
 
 // getVBIndeces of a point: the fast path for all three axes, one shared
@@ -261,22 +280,111 @@ struct RayResult {
 
 constexpr int kRayUbVoxelIndex = 1;  // TSDF_RAY_UB_VOXEL_INDEX
 constexpr int kRayUbMissingOut = 2;  // TSDF_RAY_UB_MISSING_OUT
-constexpr int kSkipLook = 4;         // empty-block probes in flight per lane
-constexpr int kInnerLook = 4;        // voxel reads in flight per lane
+#ifndef TSDF_SKIP_LOOK
+#define TSDF_SKIP_LOOK 4
+#endif
+#ifndef TSDF_INNER_LOOK
+#define TSDF_INNER_LOOK 4
+#endif
+constexpr int kSkipLook = TSDF_SKIP_LOOK;    // empty-block probes in flight per lane
+constexpr int kInnerLook = TSDF_INNER_LOOK;  // voxel reads in flight per lane
 
-// Interior of superblock s in block units, inset so that any point whose
-// approximate block coordinate q = p / vbl (error <= |q| 2^-22) lies within
-// [lo, hi] has all three reference block indices inside the superblock.
-__host__ __device__ __forceinline__ void set_sb_bounds(const int s[3], f3& lo, f3& hi)
+// Lower bound of the Chebyshev distance, in blocks, from block (x, y, z) to
+// every inserted block: outside the box of set superblocks the distance to
+// the box; inside it 8 (E - 1) + 1 for a distance of E >= 1 superblocks (the
+// nearest inserted block is at least E - 1 whole superblocks and one block
+// away), 1 in a set superblock.
+__host__ __device__ __forceinline__ int df_box(const RayParams& rp, int x, int y, int z, int& idx)
 {
-    float* l = &lo.x;
-    float* h = &hi.x;
-    for (int a = 0; a < 3; ++a) {
-        const float b0 = (float)(8 * s[a]), b1 = (float)(8 * s[a] + 8);
-        const float e = fmaxf(fmaxf(fabsf(b0), fabsf(b1)), 1.0f) * 0x1p-17f;
-        l[a] = b0 + e;
-        h[a] = b1 - e;
-    }
+    idx = 0;
+    if (rp.df_dim[0] == 0)
+        return 1 << 20;  // nothing inserted
+    const int lx = rp.df_lo[0] * 8, ly = rp.df_lo[1] * 8, lz = rp.df_lo[2] * 8;
+    const int hx = (rp.df_lo[0] + rp.df_dim[0]) * 8 - 1, hy = (rp.df_lo[1] + rp.df_dim[1]) * 8 - 1,
+              hz = (rp.df_lo[2] + rp.df_dim[2]) * 8 - 1;
+    int d = 0;
+    d = lx - x > d ? lx - x : d;
+    d = x - hx > d ? x - hx : d;
+    d = ly - y > d ? ly - y : d;
+    d = y - hy > d ? y - hy : d;
+    d = lz - z > d ? lz - z : d;
+    d = z - hz > d ? z - hz : d;
+    if (d == 0)
+        idx = ((x >> 3) - rp.df_lo[0]) + rp.df_dim[0] * (((y >> 3) - rp.df_lo[1]) + rp.df_dim[1] * ((z >> 3) - rp.df_lo[2]));
+    return d;
+}
+
+__host__ __device__ __forceinline__ int df_dist(int dbox, int e)
+{
+    return dbox > 0 ? dbox : (e >= 1 ? 8 * (e - 1) + 1 : 1);
+}
+
+__host__ __device__ __forceinline__ int free_dist(const RayParams& rp, int x, int y, int z)
+{
+    int idx;
+    const int d = df_box(rp, x, y, z, idx);
+    return df_dist(d, d == 0 ? (int)rp.df[idx] : 0);
+}
+
+// Bit of superblock (sx, sy, sz) (superblock coordinates) in the bitmap.
+__host__ __device__ __forceinline__ bool sb_bit(const uint32_t* bits, int sx, int sy, int sz)
+{
+    const int k = (sx + kSbHalf) + kSbSide * ((sy + kSbHalf) + kSbSide * (sz + kSbHalf));
+    return ((bits[k >> 5] >> (k & 31)) & 1u) != 0;
+}
+
+// The distance field of the box [lo, lo + dim) of superblocks, capped at
+// kSbdfMax, by the separable chessboard transform: along x the distance to
+// the nearest set bit of the row, then D_y = min_dy max(|dy|, D_x), then
+// D_z = min_dz max(|dz|, D_y) (windows of +-kSbdfMax).  Host build for the CPU
+// harness; k_sbdf computes the same values on the GPU.  tmp: as large as out.
+inline void sbdf_build_host(const uint32_t* sb_bits, const int lo[3], const int dim[3], uint8_t* out, uint8_t* tmp)
+{
+    const int nx = dim[0], ny = dim[1], nz = dim[2];
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+            uint8_t* row = out + (size_t)nx * (y + (size_t)ny * z);
+            int d = kSbdfMax;
+            for (int x = 0; x < nx; ++x) {
+                d = sb_bit(sb_bits, x + lo[0], y + lo[1], z + lo[2]) ? 0 : (d < kSbdfMax ? d + 1 : kSbdfMax);
+                row[x] = (uint8_t)d;
+            }
+            d = kSbdfMax;
+            for (int x = nx - 1; x >= 0; --x) {
+                d = row[x] == 0 ? 0 : (d < kSbdfMax ? d + 1 : kSbdfMax);
+                row[x] = (uint8_t)(row[x] < d ? row[x] : d);
+            }
+        }
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                int best = kSbdfMax;
+                for (int dy = -kSbdfMax; dy <= kSbdfMax; ++dy) {
+                    const int yy = y + dy;
+                    if (yy < 0 || yy >= ny)
+                        continue;
+                    const int a = dy < 0 ? -dy : dy;
+                    const int v = out[x + (size_t)nx * (yy + (size_t)ny * z)];
+                    const int m = a > v ? a : v;
+                    best = m < best ? m : best;
+                }
+                tmp[x + (size_t)nx * (y + (size_t)ny * z)] = (uint8_t)best;
+            }
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                int best = kSbdfMax;
+                for (int dz = -kSbdfMax; dz <= kSbdfMax; ++dz) {
+                    const int zz = z + dz;
+                    if (zz < 0 || zz >= nz)
+                        continue;
+                    const int a = dz < 0 ? -dz : dz;
+                    const int v = tmp[x + (size_t)nx * (y + (size_t)ny * zz)];
+                    const int m = a > v ? a : v;
+                    best = m < best ? m : best;
+                }
+                out[x + (size_t)nx * (y + (size_t)ny * z)] = (uint8_t)best;
+            }
 }
 
 template <bool kStats>
@@ -307,6 +415,112 @@ __host__ __device__ __forceinline__ f3 sel3(bool c, f3 a, f3 b) { return {c ? a.
 // Steps are taken with selects rather than branches, so a wavefront stays
 // on one instruction stream.  Reads that lie beyond the point where the
 // reference stops are discarded.
+// One probing run of the march: the lookup at rv, then, while it misses and
+// l < dmax, steps rv += delta, l += dl, each followed by a lookup -- the
+// reference's loops at GPUVoxelGrid.cpp:589-604 (delta = vbl dir) and
+// :610-627 (delta = d_trunc dir), with two changes of schedule that keep every
+// lookup and floating-point operation in its order:
+//  - kSkipLook positions (which depend on no memory) are computed and their
+//    first hash slots loaded at once, then taken in order until one hits;
+//  - with the empty-space field (rp.use_sb), a stretch of steps whose
+//    lookups certainly miss is replayed without loading: every block within
+//    Chebyshev distance d - 1 of the block of rv is absent, and k steps (each
+//    at most vbl per axis) move the reference's block index by at most k + 3
+//    per axis, so steps 1 .. d - 4 miss.
+// Returns the brick id (>= 0) or -1; steps counts the steps taken, probes
+// every lookup of the reference (replayed ones included).
+//
+// pre_valid: the first slot word of the lookup at rv (key pre_key, bucket
+// pre_bk) was already loaded as pre_slot; a hit there costs no further load.
+template <bool kStats>
+__host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayParams& rp, const Footprint& fp, f3& rv,
+                                                  float& l, const f3 delta, float dl, int& steps, unsigned int& probes,
+                                                  bool pre_valid = false, uint32_t pre_key = 0, uint32_t pre_bk = 0,
+                                                  unsigned long long pre_slot = 0)
+{
+    const float dmax = rp.dmax;
+    int id = -1;
+    bool first = true;  // the lookup at rv itself has not been made
+    if (pre_valid) {
+        const int r0 = first_slot(g, pre_key, pre_slot);
+        if (kStats)
+            fp_key(fp, pre_key);
+        ++probes;
+        id = r0 == kProbeMore ? probe_from(g, pre_key, (pre_bk + 1) & g.mask) : r0;
+        first = false;
+        if (id >= 0)
+            return id;
+    }
+    bool have = false;  // nrep holds the replay count for the block of rv
+    int nrep = 0;
+    while (first || (id < 0 && l < dmax)) {
+        if (rp.use_sb && !first) {
+            if (!have) {
+                int cx, cy, cz;
+                vb3(rv, rp, cx, cy, cz);
+                nrep = free_dist(rp, cx, cy, cz) - 4;
+            }
+            for (int k = 0; k < nrep && l < dmax; ++k) {
+                rv = add3(rv, delta);
+                l += dl;
+                ++steps;
+                ++probes;
+            }
+            have = false;
+            if (!(l < dmax))
+                break;
+        }
+        // positions: rv itself first (on the first round), then steps
+        uint32_t key[kSkipLook], bk[kSkipLook];
+        unsigned long long sl[kSkipLook];
+        int dbl = 0, ev = 0;
+        f3 r = rv;
+#pragma unroll
+        for (int k = 0; k < kSkipLook; ++k) {
+            if (!(first && k == 0))
+                r = add3(r, delta);
+            int x, y, z;
+            vb3(r, rp, x, y, z);
+            key[k] = prime_hash(x, y, z);
+            bk[k] = mix32(key[k]) & g.mask;
+            sl[k] = g.slots[bk[k]];
+            if (k == kSkipLook - 1 && rp.use_sb) {
+                int idx;
+                dbl = df_box(rp, x, y, z, idx);
+                ev = dbl == 0 ? (int)rp.df[idx] : 0;
+            }
+        }
+        bool go = true, pend = false;
+        uint32_t pkey = 0, pbk = 0;
+#pragma unroll
+        for (int k = 0; k < kSkipLook; ++k) {
+            const int r0 = first_slot(g, key[k], sl[k]);
+            if (kStats && go)
+                fp_key(fp, key[k]);
+            const bool stp = go && !(first && k == 0);  // a step precedes this lookup
+            const float ln = l + dl;
+            rv = sel3(stp, add3(rv, delta), rv);
+            l = stp ? ln : l;
+            steps += stp ? 1 : 0;
+            probes += go ? 1u : 0u;
+            id = go ? r0 : id;
+            const bool more = go && r0 == kProbeMore;
+            pend = pend || more;
+            pkey = more ? key[k] : pkey;
+            pbk = more ? bk[k] : pbk;
+            go = go && r0 == -1 && l < dmax;
+        }
+        first = false;
+        if (rp.use_sb && go) {  // every lookup taken and missed: rv is the last position
+            nrep = df_dist(dbl, ev) - 4;
+            have = true;
+        }
+        if (pend)  // first slot held another key: finish the linear probe
+            id = probe_from(g, pkey, (pbk + 1) & g.mask);
+    }
+    return id;
+}
+
 template <typename T, bool kColor, bool kStats>
 __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const RayParams& rp, int i, int j,
                                     const Footprint& fp)
@@ -335,102 +549,35 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
     const f3 step = {vl * dir.x, vl * dir.y, vl * dir.z};     // and vl * dir
 
     bool Lp = false;
-    bool sb_ok = false;  // sb_lo / sb_hi hold an empty superblock (block units, inset)
-    f3 sb_lo = {0.f, 0.f, 0.f}, sb_hi = {0.f, 0.f, 0.f};
+    // the lookup at rv, loaded ahead by the voxel walk that left a block
+    bool pre_valid = false;
+    uint32_t pre_key = 0, pre_bk = 0;
+    unsigned long long pre_slot = 0;
     while (l < dmax) {  // outer loop; done / Fn leave by return
-        int bx, by, bz;
-        vb3(rv, rp, bx, by, bz);
-        int id = probe_count<kStats>(g, fp, bx, by, bz, out.probes);
+        // ---- the block lookup at rv and the empty-block skip ------------
+        // (GPUVoxelGrid.cpp:589-604): probe rv, then steps of vbl * dir
         int count = 0;
-        // ---- empty-block skip ----------------------------------------
-        while (id < 0 && l < dmax) {
-            // (a) steps into superblocks whose bit is clear: the probe
-            // provably misses (alias_free), so only the reference's position
-            // / length arithmetic is replayed; the bit is read once per
-            // superblock entered
-            if (rp.use_sb) {
-                while (true) {
-                    const f3 rn = add3(rv, skip);
-                    const float qx = rn.x * rp.rvbl, qy = rn.y * rp.rvbl, qz = rn.z * rp.rvbl;
-                    if (!(sb_ok && qx >= sb_lo.x && qx <= sb_hi.x && qy >= sb_lo.y && qy <= sb_hi.y &&
-                          qz >= sb_lo.z && qz <= sb_hi.z)) {
-                        int x, y, z;
-                        vb3(rn, rp, x, y, z);
-                        const int k = sb_index(x, y, z);
-                        if (k < 0 || ((g.sb_bits[k >> 5] >> (k & 31)) & 1u))
-                            break;  // not known empty: probe the hash
-                        const int sbc[3] = {x >> 3, y >> 3, z >> 3};
-                        set_sb_bounds(sbc, sb_lo, sb_hi);
-                        sb_ok = true;
-                    }
-                    rv = rn;
-                    l += vbl;
-                    ++count;
-                    ++out.probes;
-                    if (!(l < dmax))
-                        break;
-                }
-                if (!(l < dmax))
-                    break;
-            }
-            // (b) kSkipLook further steps with their first hash slots loaded
-            // at once, taken in order
-            uint32_t key[kSkipLook], bk[kSkipLook];
-            unsigned long long sl[kSkipLook];
-            f3 r = rv;
-#pragma unroll
-            for (int k = 0; k < kSkipLook; ++k) {
-                r = add3(r, skip);
-                int x, y, z;
-                vb3(r, rp, x, y, z);
-                key[k] = prime_hash(x, y, z);
-                bk[k] = mix32(key[k]) & g.mask;
-                sl[k] = g.slots[bk[k]];
-            }
-            bool go = true, pend = false;
-            uint32_t pkey = 0, pbk = 0;
-#pragma unroll
-            for (int k = 0; k < kSkipLook; ++k) {
-                const int r0 = first_slot(g, key[k], sl[k]);
-                if (kStats && go)
-                    fp_key(fp, key[k]);
-                const float ln = l + vbl;
-                rv = sel3(go, add3(rv, skip), rv);
-                l = go ? ln : l;
-                count += go ? 1 : 0;
-                out.probes += go ? 1u : 0u;
-                id = go ? r0 : id;
-                const bool more = go && r0 == kProbeMore;
-                pend = pend || more;
-                pkey = more ? key[k] : pkey;
-                pbk = more ? bk[k] : pbk;
-                go = go && r0 == -1 && ln < dmax;
-            }
-            if (pend)  // first slot held another key: finish the linear probe
-                id = probe_from(g, pkey, (pbk + 1) & g.mask);
-        }
+        int id = probe_run<kStats>(g, rp, fp, rv, l, skip, vbl, count, out.probes, pre_valid, pre_key, pre_bk,
+                                   pre_slot);
+        pre_valid = false;
         if (!(l < dmax))
             break;
         if (count > 0) {
-            // back off and re-advance by d_trunc, GPUVoxelGrid.cpp:610-627
+            // back off and re-advance by d_trunc, GPUVoxelGrid.cpp:610-627:
+            // probe the backed-off position, then steps of d_trunc * dir
             const float back = rp.back;
             rv.x -= back * dir.x;
             rv.y -= back * dir.y;
             rv.z -= back * dir.z;
             l -= back;
-            vb3(rv, rp, bx, by, bz);
-            id = probe_count<kStats>(g, fp, bx, by, bz, out.probes);
-            while (id < 0 && l < dmax) {
-                rv.x += dt * dir.x;
-                rv.y += dt * dir.y;
-                rv.z += dt * dir.z;
-                l += dt;
-                vb3(rv, rp, bx, by, bz);
-                id = probe_count<kStats>(g, fp, bx, by, bz, out.probes);
-            }
+            const f3 dstep = {dt * dir.x, dt * dir.y, dt * dir.z};
+            int steps = 0;
+            id = probe_run<kStats>(g, rp, fp, rv, l, dstep, dt, steps, out.probes);
         }
         if (!(l < dmax))
             break;
+        int bx, by, bz;
+        vb3(rv, rp, bx, by, bz);
         // ---- voxel walk inside block (bx, by, bz) ----------------------
         const int cur = id;
         const T* brick = (const T*)g.payload + (size_t)cur * (2 * kVPB);
@@ -442,24 +589,44 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
             // outside [0, 8) (reference UB), else x + 8y + 64z
             int vk[kInnerLook];
             float sk[kInnerLook], wk[kInnerLook];
+            // the first position of this window or the next one outside the
+            // block: its lookup is loaded with the voxels, for the outer loop
+            bool e_found = false;
+            int ex = 0, ey = 0, ez = 0;
             {
                 f3 r = rv;
                 float ll = l;
                 bool same = l < dmax;
                 int wx = vx, wy = vy, wz = vz;
 #pragma unroll
-                for (int k = 0; k < kInnerLook; ++k) {
+                for (int k = 0; k <= kInnerLook; ++k) {
                     if (k > 0) {
                         r = add3(r, step);
                         ll += vl;
                         int qx, qy, qz;
                         vb3(r, rp, qx, qy, qz);
-                        same = same && qx == bx && qy == by && qz == bz && ll < dmax;
-                        vin3(r, bx, by, bz, rp, wx, wy, wz);
+                        const bool inb = qx == bx && qy == by && qz == bz;
+                        const bool leave = same && !inb;
+                        ex = leave ? qx : ex;
+                        ey = leave ? qy : ey;
+                        ez = leave ? qz : ez;
+                        e_found = e_found || leave;
+                        same = same && inb && ll < dmax;
+                        if (k < kInnerLook)
+                            vin3(r, bx, by, bz, rp, wx, wy, wz);
                     }
-                    vk[k] = !same ? -2 : (vin_ok(wx, wy, wz) ? wx + kVPE * wy + kVPE * kVPE * wz : -1);
-                    ld_pair(brick, vk[k] >= 0 ? vk[k] : 0, sk[k], wk[k]);
+                    if (k < kInnerLook) {
+                        vk[k] = !same ? -2 : (vin_ok(wx, wy, wz) ? wx + kVPE * wy + kVPE * kVPE * wz : -1);
+                        ld_pair(brick, vk[k] >= 0 ? vk[k] : 0, sk[k], wk[k]);
+                    }
                 }
+            }
+            uint32_t e_key = 0, e_bk = 0;
+            unsigned long long e_slot = 0;
+            if (e_found) {
+                e_key = prime_hash(ex, ey, ez);
+                e_bk = mix32(e_key) & g.mask;
+                e_slot = g.slots[e_bk];
             }
             bool stop = false;
             int ending = 0;  // 1 left the block / range, 2 UB index, 3 Fn, 4 crossing
@@ -489,13 +656,25 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
             if (!stop) {  // walked past the window: position, block and index of rv
                 int nx, ny, nz;
                 vb3(rv, rp, nx, ny, nz);
-                if (nx != bx || ny != by || nz != bz || !(l < dmax))
-                    break;  // the inner loop test fails
+                if (nx != bx || ny != by || nz != bz || !(l < dmax)) {
+                    // the inner loop test fails; rv is the window's next
+                    // position, whose lookup was loaded if it left the block
+                    pre_valid = e_found;
+                    pre_key = e_key;
+                    pre_bk = e_bk;
+                    pre_slot = e_slot;
+                    break;
+                }
                 vin3(rv, bx, by, bz, rp, vx, vy, vz);
                 continue;
             }
-            if (ending == 1)
-                break;  // back to the outer loop
+            if (ending == 1) {  // back to the outer loop from the first position that left
+                pre_valid = e_found;
+                pre_key = e_key;
+                pre_bk = e_bk;
+                pre_slot = e_slot;
+                break;
+            }
             if (ending == 2) {
                 out.code = -kRayUbVoxelIndex;
                 return out;

@@ -97,6 +97,22 @@ extern "C" int march_host(int H, int W, float fx, float fy, float cx, float cy, 
         }
         rp.use_sb = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
     }
+    std::vector<uint8_t> df, df_tmp;
+    rp.df = nullptr;
+    for (int a = 0; a < 3; ++a)
+        rp.df_lo[a] = rp.df_dim[a] = 0;
+    if (rp.use_sb && n > 0) {  // the distance field of the superblock bits (as k_sbdf builds it)
+        size_t vol = 1;
+        for (int a = 0; a < 3; ++a) {
+            rp.df_lo[a] = bmin[a] >> 3;
+            rp.df_dim[a] = (bmax[a] >> 3) - rp.df_lo[a] + 1;
+            vol *= (size_t)rp.df_dim[a];
+        }
+        df.resize(vol);
+        df_tmp.resize(vol);
+        sbdf_build_host(sb.data(), rp.df_lo, rp.df_dim, df.data(), df_tmp.data());
+        rp.df = df.data();
+    }
     *certified = rp.use_sb;
     Footprint fp;
     std::memset(&fp, 0, sizeof(fp));

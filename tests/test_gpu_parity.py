@@ -84,6 +84,17 @@ def test_plain_small_full_size(gpu):
     fp = g.raycast_footprint(poses[7])
     r = o.raycast(poses[7], stats=True)["counts"]
     assert fp["u_vox"] == r["u_vox"] and fp["u_blk"] == r["u_blk"], (fp, r)
+    # the empty-space replay keeps every point and count
+    g.set_tuning(raycast_superblocks=1)
+    before = g.stats()
+    samples = probes = 0
+    for k in (0, 7, 15, 22):
+        r = assert_raycast_equal(g, o, poses[k])
+        samples += r["counts"]["samples"]
+        probes += r["counts"]["probes"]
+    after = g.stats()
+    assert after["ray_samples"] - before["ray_samples"] == samples
+    assert after["ray_probes"] - before["ray_probes"] == probes
 
 
 def test_deintegrate_sequence(gpu):
@@ -144,6 +155,23 @@ def test_room_large_frames_with_weight_cap(gpu):
     eg, _ = assert_grids_equal(g, o)
     assert eg["w"].max() == 3.0
     assert_raycast_equal(g, o, poses[5])
+    # with the empty-space replay: the same points and the same (reference)
+    # probe and sample counts, fewer actual probes
+    g.set_tuning(raycast_superblocks=1)
+    for k in (2, 5, 7):
+        before = g.stats()
+        r = assert_raycast_equal(g, o, poses[k])
+        after = g.stats()
+        assert after["ray_probes"] - before["ray_probes"] == r["counts"]["probes"]
+        assert after["ray_samples"] - before["ray_samples"] == r["counts"]["samples"]
+    # new superblocks after the field was built: it is rebuilt before the next raycast
+    far = synth.pose_f32(synth.look_at((-1.5, -1.2, 1.2), (2.0, 1.5, 0.5)))
+    d_far = synth.frames(synth.scene_room(seed=1), [far.astype(np.float64)], dict(cfg, intr=intr), seed=9,
+                         H=cfg["H"], W=cfg["W"])[0]
+    g.integrate(None, dev_u16(d_far), far)
+    o.integrate(d_far, far)
+    assert_raycast_equal(g, o, far)
+    assert_raycast_equal(g, o, poses[3])
 
 
 def test_batched_reintegration_equals_plain_calls(gpu):
