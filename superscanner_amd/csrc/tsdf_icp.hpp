@@ -215,15 +215,38 @@ __global__ __launch_bounds__(kThreads) void k_icp_moments(MomentArgs a)
     }
 }
 
-// partial sums of the blocks, in block order
-__global__ void k_icp_moments_final(const double* partial, int n_blocks, double* out)
+// The blocks' partial sums: one wavefront per moment, each lane summing the
+// blocks lane, lane + 64, ... in order, then the fixed wavefront tree.
+constexpr int kIcpFinalThreads = 256;
+
+__global__ __launch_bounds__(kIcpFinalThreads) void k_icp_moments_final(const double* partial, int n_blocks,
+                                                                        double* out)
 {
-    if (threadIdx.x < kIcpMoments) {
+    const int lane = threadIdx.x & 63;
+    for (int q = threadIdx.x >> 6; q < kIcpMoments; q += kIcpFinalThreads / 64) {
         double s = 0.0;
-        for (int b = 0; b < n_blocks; ++b)
-            s += partial[(size_t)b * kIcpMoments + threadIdx.x];
-        out[threadIdx.x] = s;
+        for (int b = lane; b < n_blocks; b += 64)
+            s += partial[(size_t)b * kIcpMoments + q];
+        s = wave_sum_f64(s);
+        if (lane == 0)
+            out[q] = s;
     }
+}
+
+// The sums into buf + kIcpBlocks * kIcpMoments (device), with buf's first
+// kIcpBlocks * kIcpMoments doubles as the partial sums.
+void launch_icp_moments(const float* x, const float* xn, const double* T16, unsigned int n, double* buf, hipStream_t s)
+{
+    MomentArgs a;
+    a.x = x;
+    a.xn = xn;
+    for (int k = 0; k < 12; ++k)
+        a.T[k] = T16[k];
+    a.n = n;
+    a.partial = buf;
+    hipLaunchKernelGGL(k_icp_moments, dim3(kIcpBlocks), dim3(kThreads), 0, s, a);
+    hipLaunchKernelGGL(k_icp_moments_final, dim3(1), dim3(kIcpFinalThreads), 0, s, buf, kIcpBlocks,
+                       buf + (size_t)kIcpBlocks * kIcpMoments);
 }
 
 int icp_grid(size_t n)
@@ -424,16 +447,7 @@ tsdf_status tsdf_icp_moments(const float* x_points, const float* x_nearest, cons
     hipStream_t s = (hipStream_t)stream;
     double* buf = nullptr;
     HIPC(hipMalloc(&buf, (size_t)(kIcpBlocks + 1) * kIcpMoments * sizeof(double)));
-    MomentArgs a;
-    a.x = x_points;
-    a.xn = x_nearest;
-    for (int k = 0; k < 12; ++k)
-        a.T[k] = Txy[k];
-    a.n = (unsigned int)height * (unsigned int)width;
-    a.partial = buf;
-    hipLaunchKernelGGL(k_icp_moments, dim3(kIcpBlocks), dim3(kThreads), 0, s, a);
-    hipLaunchKernelGGL(k_icp_moments_final, dim3(1), dim3(64), 0, s, buf, kIcpBlocks,
-                       buf + (size_t)kIcpBlocks * kIcpMoments);
+    launch_icp_moments(x_points, x_nearest, Txy, (unsigned int)height * (unsigned int)width, buf, s);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipMemcpyAsync(moments, buf + (size_t)kIcpBlocks * kIcpMoments, kIcpMoments * sizeof(double),
@@ -457,8 +471,17 @@ tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const doubl
         return fail(TSDF_BAD_ARG, "ty is singular");
     mat4_mul(tyi, tx, T);  // Txy = ty^-1 tx, kokkosICP.cpp:52
     const size_t n = (size_t)cam->height * cam->width;
+    hipStream_t s = (hipStream_t)stream;
     float* buf = nullptr;
+    double* mom = nullptr;
+    double* mom_host = nullptr;
     HIPC(hipMalloc(&buf, 4 * n * 3 * sizeof(float)));
+    if (hipMalloc(&mom, (size_t)(kIcpBlocks + 1) * kIcpMoments * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&mom_host, kIcpMoments * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        (void)hipFree(buf);
+        (void)hipFree(mom);
+        return fail(TSDF_HIP_ERROR, "icp workspace allocation failed");
+    }
     float *x = buf, *y = buf + 3 * n, *yn = buf + 6 * n, *xn = buf + 9 * n;
     tsdf_status r = tsdf_depth_to_points(depth1, cam, x, stream);
     if (r == TSDF_OK)
@@ -472,9 +495,18 @@ tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const doubl
         for (int k = 0; k < 16; ++k)
             Tf[k] = (float)T[k];
         r = tsdf_icp_associate(x, y, yn, Tf, cam, p, xn, stream);
-        double m[kIcpMoments];
-        if (r == TSDF_OK)
-            r = tsdf_icp_moments(x, xn, T, cam->height, cam->width, m, stream);
+        if (r == TSDF_OK) {
+            launch_icp_moments(x, xn, T, (unsigned int)n, mom, s);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(mom_host, mom + (size_t)kIcpBlocks * kIcpMoments, kIcpMoments * sizeof(double),
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(s);
+            if (e != hipSuccess)
+                r = fail(TSDF_HIP_ERROR, std::string("icp moments: ") + hipGetErrorString(e));
+        }
+        const double* m = mom_host;
         if (r != TSDF_OK || m[1] == 0.0)
             break;  // no correspondence: the reference's sums turn to NaN and nothing improves
         const double nc = m[1];
@@ -506,6 +538,8 @@ tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const doubl
         }
     }
     (void)hipFree(buf);
+    (void)hipFree(mom);
+    (void)hipHostFree(mom_host);
     if (r != TSDF_OK)
         return r;
     for (int k = 0; k < 16; ++k)
