@@ -102,50 +102,156 @@ struct IcpArgs {
 // distance below max_dist wins (point-to-point: |y - px|, out = y;
 // point-to-plane: |(y - px) . n_y| over y with n_y.x != 0, out = px + d n_y).
 // (0, 0, 0) where nothing qualifies.
-__global__ __launch_bounds__(kThreads) void k_icp_associate(IcpArgs a)
+//
+// One workgroup per 16 x 16 pixels of x.  The union of its pixels' windows is
+// staged in LDS (y, and the normals for point to plane) when it holds at most
+// kIcpLds candidates; the scan then reads LDS, in the same order, else global
+// memory.  kIcpLds = 0 always scans global memory.
+#ifndef TSDF_ICP_LDS
+#define TSDF_ICP_LDS 1024
+#endif
+constexpr int kIcpTile = 16;
+constexpr int kIcpLds = TSDF_ICP_LDS;  // staged candidates per workgroup
+constexpr int kIcpLdsAlloc = kIcpLds > 0 ? kIcpLds : 1;
+
+// candidate (r, c): from the staged box (origin r0, c0, width bw) or global memory
+template <bool kLds>
+__device__ __forceinline__ f3 icp_cand(const float* lx, const float* ly, const float* lz, int r, int c, int r0,
+                                      int c0, int bw, const float* g, unsigned int q)
 {
-    const unsigned int n = (unsigned int)a.H * (unsigned int)a.W;
-    for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        f3 best{0.0f, 0.0f, 0.0f};
-        const f3 xp = ld3(a.x, k);
-        if (xp.z != 0.0f) {
-            const f3 px = xform(xp, a.T);
-            const float cxi = px.y * a.cam.fy / px.z + a.cam.ci;
-            const float cxj = px.x * a.cam.fx / px.z + a.cam.cj;
-            const int ci = f2i(cxi), cj = f2i(cxj);
-            const int ns = a.n_search;
-            float smallest = 1e6f;
-            if (ci > ns && ci < a.H - ns && cj > ns && cj < a.W - ns) {
-                for (int r = ci - ns; r < ci + ns; ++r) {
-                    for (int c = cj - ns; c < cj + ns; ++c) {
-                        const unsigned int q = (unsigned int)(r * a.W + c);
-                        const f3 py = ld3(a.y, q);
-                        if (a.type == TSDF_ICP_POINT_TO_POINT) {
-                            if (py.z != 0.0f) {
-                                const float dist = len3(sub3(py, px));
-                                if (dist < smallest && dist < a.max_dist) {
-                                    smallest = dist;
-                                    best = py;
-                                }
-                            }
-                        } else {
-                            const f3 nv = ld3(a.yn, q);
-                            if (py.z != 0.0f && nv.x != 0.0f) {
-                                const f3 dv = sub3(py, px);
-                                const float d = dv.x * nv.x + dv.y * nv.y + dv.z * nv.z;
-                                const float dist = fabsf(d);
-                                if (dist < smallest && dist < a.max_dist) {
-                                    smallest = dist;
-                                    best = f3{px.x + d * nv.x, px.y + d * nv.y, px.z + d * nv.z};
-                                }
-                            }
-                        }
+    if (kLds) {
+        const int o = (r - r0) * bw + (c - c0);
+        return f3{lx[o], ly[o], lz[o]};
+    }
+    return ld3(g, q);
+}
+
+template <bool kLds, bool kPlane>
+__device__ __forceinline__ f3 icp_scan(const IcpArgs& a, f3 px, int ci, int cj, const float* ly_x, const float* ly_y,
+                                       const float* ly_z, const float* ln_x, const float* ln_y, const float* ln_z,
+                                       int r0, int c0, int bw)
+{
+    f3 best{0.0f, 0.0f, 0.0f};
+    const int ns = a.n_search;
+    float smallest = 1e6f;
+    // squared length of the current best: sqrt is monotone, so a candidate
+    // whose squared length is not smaller cannot pass `dist < smallest`, and
+    // only the others need the sqrt
+    float best_sq = __builtin_inff();
+    for (int r = ci - ns; r < ci + ns; ++r) {
+        for (int c = cj - ns; c < cj + ns; ++c) {
+            const unsigned int q = (unsigned int)(r * a.W + c);
+            const f3 py = icp_cand<kLds>(ly_x, ly_y, ly_z, r, c, r0, c0, bw, a.y, q);
+            if (!kPlane) {
+                const f3 dv = sub3(py, px);
+                const float sq = dv.x * dv.x + dv.y * dv.y + dv.z * dv.z;
+                if (py.z != 0.0f && sq < best_sq) {
+                    const float dist = sqrtf(sq);
+                    if (dist < smallest && dist < a.max_dist) {
+                        smallest = dist;
+                        best_sq = sq;
+                        best = py;
+                    }
+                }
+            } else {
+                const f3 nv = icp_cand<kLds>(ln_x, ln_y, ln_z, r, c, r0, c0, bw, a.yn, q);
+                if (py.z != 0.0f && nv.x != 0.0f) {
+                    const f3 dv = sub3(py, px);
+                    const float d = dv.x * nv.x + dv.y * nv.y + dv.z * nv.z;
+                    const float dist = fabsf(d);
+                    if (dist < smallest && dist < a.max_dist) {
+                        smallest = dist;
+                        best = f3{px.x + d * nv.x, px.y + d * nv.y, px.z + d * nv.z};
                     }
                 }
             }
         }
-        st3(a.out, k, best);
     }
+    return best;
+}
+
+template <bool kPlane>
+__global__ __launch_bounds__(kIcpTile * kIcpTile) void k_icp_associate(IcpArgs a)
+{
+    __shared__ float sy[3][kIcpLdsAlloc];
+    __shared__ float sn[kPlane ? 3 : 1][kPlane ? kIcpLdsAlloc : 1];
+    __shared__ int box[4];  // row min, row max (exclusive), col min, col max (exclusive)
+    const int tiles_x = (a.W + kIcpTile - 1) / kIcpTile;
+    const int i = (int)(blockIdx.x / tiles_x) * kIcpTile + (int)(threadIdx.x / kIcpTile);
+    const int j = (int)(blockIdx.x % tiles_x) * kIcpTile + (int)(threadIdx.x % kIcpTile);
+    const bool in = i < a.H && j < a.W;
+    const unsigned int k = in ? (unsigned int)(i * a.W + j) : 0u;
+    const int ns = a.n_search;
+    bool act = false;
+    f3 px{0.0f, 0.0f, 0.0f};
+    int ci = 0, cj = 0;
+    if (in) {
+        const f3 xp = ld3(a.x, k);
+        if (xp.z != 0.0f) {
+            px = xform(xp, a.T);
+            const float cxi = px.y * a.cam.fy / px.z + a.cam.ci;
+            const float cxj = px.x * a.cam.fx / px.z + a.cam.cj;
+            ci = f2i(cxi);
+            cj = f2i(cxj);
+            act = ci > ns && ci < a.H - ns && cj > ns && cj < a.W - ns;
+        }
+    }
+    bool stage = false;
+    int r0 = 0, c0 = 0, bw = 0;
+    if (kIcpLds > 0) {
+        if (threadIdx.x == 0) {
+            box[0] = 0x7FFFFFFF;
+            box[1] = (int)0x80000000u;
+            box[2] = 0x7FFFFFFF;
+            box[3] = (int)0x80000000u;
+        }
+        __syncthreads();
+        if (act) {
+            atomicMin(&box[0], ci - ns);
+            atomicMax(&box[1], ci + ns);
+            atomicMin(&box[2], cj - ns);
+            atomicMax(&box[3], cj + ns);
+        }
+        __syncthreads();
+        // a workgroup without an active pixel leaves the sentinels in place:
+        // compare before subtracting (their difference would wrap around).
+        // An active pixel has 1 <= ci - ns and ci + ns <= H - 1 (columns
+        // alike), so the box lies inside the image.
+        const bool any = box[0] < box[1] && box[2] < box[3];
+        r0 = any ? box[0] : 0;
+        c0 = any ? box[2] : 0;
+        const int bh = any ? box[1] - box[0] : 0;
+        bw = any ? box[3] - box[2] : 0;
+        stage = any && bh * bw <= kIcpLds;
+        if (stage) {
+            for (int t = threadIdx.x; t < bh * bw; t += kIcpTile * kIcpTile) {
+                const int r = r0 + t / bw, c = c0 + t % bw;
+                const unsigned int q = (unsigned int)(r * a.W + c);
+                const f3 v = ld3(a.y, q);
+                sy[0][t] = v.x;
+                sy[1][t] = v.y;
+                sy[2][t] = v.z;
+                if (kPlane) {
+                    const f3 nv = ld3(a.yn, q);
+                    sn[0][t] = nv.x;
+                    sn[kPlane ? 1 : 0][t] = nv.y;
+                    sn[kPlane ? 2 : 0][t] = nv.z;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    f3 best{0.0f, 0.0f, 0.0f};
+    if (act) {
+        if (stage)
+            best = icp_scan<true, kPlane>(a, px, ci, cj, sy[0], sy[1], sy[2], sn[0], sn[kPlane ? 1 : 0],
+                                          sn[kPlane ? 2 : 0], r0, c0, bw);
+        else
+            best = icp_scan<false, kPlane>(a, px, ci, cj, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0,
+                                           0);
+    }
+    if (in)
+        st3(a.out, k, best);
 }
 
 // Sums of one ICP iteration, in double: [0] points of x (z != 0), [1]
@@ -432,8 +538,11 @@ tsdf_status tsdf_icp_associate(const float* x_points, const float* y_points, con
     a.max_dist = p->max_dist;
     a.type = p->type;
     a.out = x_nearest;
-    const size_t n = (size_t)a.H * a.W;
-    hipLaunchKernelGGL(k_icp_associate, dim3(icp_grid(n)), dim3(kThreads), 0, (hipStream_t)stream, a);
+    const int tiles = ((a.W + kIcpTile - 1) / kIcpTile) * ((a.H + kIcpTile - 1) / kIcpTile);
+    if (a.type == TSDF_ICP_POINT_TO_PLANE)
+        hipLaunchKernelGGL(k_icp_associate<true>, dim3(tiles), dim3(kIcpTile * kIcpTile), 0, (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(k_icp_associate<false>, dim3(tiles), dim3(kIcpTile * kIcpTile), 0, (hipStream_t)stream, a);
     HIPC(hipGetLastError());
     return TSDF_OK;
 }

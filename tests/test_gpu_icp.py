@@ -3,7 +3,8 @@ against the oracle (oracle/icp_oracle.c, pinned by test_icp_oracle.py):
 
   * depth -> points, points -> normals and the association of both types are
     bit-identical to the oracle (NaN normals compared as NaN), on rendered
-    frames with holes, at 640 x 480 and at an odd size;
+    frames with holes, at 640 x 480 and at an odd size, through both the
+    LDS-staged scan and the global-memory one;
   * the iteration sums agree with the oracle's reference-order sums (counts
     exact, centroids and the centred cross-covariance to 1e-9 relative, the
     RMSE, which the reference sums in float, to 1e-5 relative);
@@ -56,7 +57,12 @@ def test_points_normals_association_bit_exact(gpu, size):
     turn = np.eye(4)
     turn[:3, :3] = synth.axis_angle((1.0, 0.2, -0.4), np.deg2rad(4.0))
     for T in (np.eye(4), Txy, Txy @ turn):
-        for t, ns, md in ((icp.PointToPoint, 5, 0.075), (icp.PointToPlane, 5, 0.075), (icp.PointToPoint, 3, 0.02)):
+        # n_search 3 and 5 stage every 16 x 16 tile's window box in LDS
+        # (at most 1024 candidates); at 8 (a box of 32 x 32 and more) the
+        # widely spread tiles of the turned transform scan global memory, and
+        # at 12 most tiles do
+        for t, ns, md in ((icp.PointToPoint, 5, 0.075), (icp.PointToPlane, 5, 0.075), (icp.PointToPoint, 3, 0.02),
+                          (icp.PointToPlane, 8, 0.075), (icp.PointToPoint, 12, 0.1)):
             got = icp.associate(x, y, yn, T, intr, icp_type=t, n_search=ns, max_dist=md).cpu().numpy()
             exp = oicp.associate(ox, oy, oyn, T, intr, icp_type=t, n_search=ns, max_dist=md)
             assert (exp[..., 2] != 0).mean() > 0.1
