@@ -124,6 +124,31 @@ __device__ __forceinline__ unsigned int wave_sum(unsigned int v)
     return v;
 }
 
+__device__ __forceinline__ unsigned int workgroup_index()
+{
+    return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
+// Lane `who` (mod kStatLanes) of statistics counter `id`.
+__device__ __forceinline__ unsigned long long* stat_lane(const GridView& g, int id, unsigned int who)
+{
+    return g.stat + ((size_t)id * kStatLanes + who % kStatLanes) * kStatStride;
+}
+
+// Widens the inserted-block box to (x, y, z).  The box only grows, so a
+// coordinate inside a value read earlier is inside the current box too and
+// needs no atomic (most inserts fall inside it).
+__device__ __forceinline__ void grow_bbox(Ctrl* c, int x, int y, int z)
+{
+    const int v[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a) {
+        if (v[a] < __hip_atomic_load(&c->bbox_min[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&c->bbox_min[a], v[a]);
+        if (v[a] > __hip_atomic_load(&c->bbox_max[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&c->bbox_max[a], v[a]);
+    }
+}
+
 // Finds `key` or inserts it.  The thread whose CAS claims an empty slot
 // takes the next brick id and publishes it; concurrent finders of the same
 // key only need the slot index.  Returns the slot or -1 if the table is full.
@@ -141,7 +166,7 @@ __device__ int find_or_insert(const GridView& g, uint32_t key, int x, int y, int
                 if (id < g.n_bricks) {
                     g.coords[id] = make_int4(x, y, z, 1);
                     idv = id;
-                    atomicAdd(&g.ctrl->total_new, 1ull);
+                    atomicAdd(stat_lane(g, kStNew, workgroup_index()), 1ull);
                 }
                 {
                     // every inserted key counts for the raycast's empty-space
@@ -149,12 +174,7 @@ __device__ int find_or_insert(const GridView& g, uint32_t key, int x, int y, int
                     const int sb = sb_index(x, y, z);
                     if (sb >= 0 && !(atomicOr(&g.sb_bits[sb >> 5], 1u << (sb & 31)) & (1u << (sb & 31))))
                         atomicAdd(&g.ctrl->sb_version, 1u);  // a new superblock: the distance field is stale
-                    atomicMin(&g.ctrl->bbox_min[0], x);
-                    atomicMin(&g.ctrl->bbox_min[1], y);
-                    atomicMin(&g.ctrl->bbox_min[2], z);
-                    atomicMax(&g.ctrl->bbox_max[0], x);
-                    atomicMax(&g.ctrl->bbox_max[1], y);
-                    atomicMax(&g.ctrl->bbox_max[2], z);
+                    grow_bbox(g.ctrl, x, y, z);
                 }
                 if (id >= g.n_bricks)
                     atomicAdd(&g.ctrl->overflow, 1u);
@@ -257,6 +277,8 @@ __global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
             op_mask[i] = 0u;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < kSbWords; i += stride)
         g.sb_bits[i] = 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < kStatWords; i += stride)
+        g.stat[i] = 0ull;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Ctrl z;
         memset(&z, 0, sizeof(z));
@@ -397,7 +419,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
         atomicAdd(&s_valid, nvalid);
     __syncthreads();
     if (threadIdx.x == 0 && s_valid)
-        atomicAdd(&g.ctrl->total_valid, (unsigned long long)s_valid);
+        atomicAdd(stat_lane(g, kStValid, workgroup_index()), (unsigned long long)s_valid);
 
     // compact the tile's distinct blocks, then claim them all in one round
     // (one dependent chain of global accesses per tile rather than one per
@@ -665,6 +687,25 @@ __device__ __forceinline__ WaveRange wave_range(unsigned int count)
 // ---------------------------------------------------------------------------
 constexpr int kMetaChunk = 16;
 
+// Adds every thread's `v` to statistics counter `id` with one atomic per
+// workgroup, on the workgroup's lane (all threads of the workgroup must call
+// it).  The wavefronts of a persistent update launch finish close together.
+__device__ __forceinline__ void add_per_workgroup(const GridView& g, int id, unsigned int v)
+{
+    __shared__ unsigned int part[kThreads / 64];
+    v = wave_sum(v);
+    if (__lane_id() == 0)
+        part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+        for (int k = 0; k < kThreads / 64; ++k)
+            t += part[k];
+        if (t)
+            atomicAdd(stat_lane(g, id, workgroup_index()), (unsigned long long)t);
+    }
+}
+
 // Resolves touched[t0 .. t1) (at most kMetaChunk entries) into LDS:
 // (id | fresh << 31, x, y, z), id = -1 when the slot's allocation failed.
 __device__ __forceinline__ void load_meta(const GridView& g, unsigned int t0, unsigned int t1, int4* meta)
@@ -721,9 +762,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, i
             store_lane<T, kColor>(g, id, lane, fresh ? 0xFFu : upd, v);
         }
     }
-    nupd = wave_sum(nupd);
-    if (lane == 0 && nupd)
-        atomicAdd(&g.ctrl->total_updated, (unsigned long long)nupd);
+    add_per_workgroup(g, kStUpdated, nupd);
 }
 
 // ---------------------------------------------------------------------------
@@ -780,9 +819,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_batch(GridView g, const 
         if (lane < n_words)  // every lane has read the mask words above
             op_mask[(size_t)slot * kMaskWords + lane] = 0u;
     }
-    nupd = wave_sum(nupd);
-    if (lane == 0 && nupd)
-        atomicAdd(&g.ctrl->total_updated, (unsigned long long)nupd);
+    add_per_workgroup(g, kStUpdated, nupd);
 }
 
 // ---------------------------------------------------------------------------
@@ -979,11 +1016,14 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     bad = wave_sum(bad);
     samples = wave_sum(samples);
     probes = wave_sum(probes);
+    // one atomic per wavefront and counter, on the wavefront's lane: the
+    // wavefronts end at very different times, so none waits for the others
     if (lane == 0) {
-        if (hit) atomicAdd(&g.ctrl->ray_hits, (unsigned long long)hit);
-        if (bad) atomicAdd(&g.ctrl->ray_ub, (unsigned long long)bad);
-        if (samples) atomicAdd(&g.ctrl->ray_samples, (unsigned long long)samples);
-        if (probes) atomicAdd(&g.ctrl->ray_probes, (unsigned long long)probes);
+        const unsigned int who = blockIdx.x * kRayWaves + (unsigned int)wv;
+        if (hit) atomicAdd(stat_lane(g, kStRayHits, who), (unsigned long long)hit);
+        if (bad) atomicAdd(stat_lane(g, kStRayUb, who), (unsigned long long)bad);
+        if (samples) atomicAdd(stat_lane(g, kStRaySamples, who), (unsigned long long)samples);
+        if (probes) atomicAdd(stat_lane(g, kStRayProbes, who), (unsigned long long)probes);
     }
 }
 
@@ -1056,6 +1096,8 @@ struct tsdf_grid {
     float4* ptmap;
     Ctrl* ctrl;
     Ctrl* ctrl_host;  // pinned
+    unsigned long long* stat;       // statistics counters, kStatWords
+    unsigned long long* stat_host;  // pinned
     // batch buffers (allocated on first batched call)
     uint32_t* op_mask;
     float4* ptmap_batch;
@@ -1137,6 +1179,7 @@ GridView view(const tsdf_grid* g)
     v.sbdf = g->sbdf;
     v.sbdf_info = g->sbdf_info;
     v.ctrl = g->ctrl;
+    v.stat = g->stat;
     v.mask = g->p.n_buckets - 1u;
     v.n_bricks = g->p.n_bricks;
     v.vl = g->p.voxel_length;
@@ -1482,17 +1525,22 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
     alloc((void**)&g->sbdf_info, sizeof(SbdfInfo));
     alloc((void**)&g->ptmap, (size_t)p->height * p->width * sizeof(float4));
     alloc((void**)&g->ctrl, sizeof(Ctrl));
+    alloc((void**)&g->stat, kStatWords * sizeof(unsigned long long));
     if (st == TSDF_OK) {
         // no distance field yet; counters start from zero
         hipError_t e = hipMemset(g->sbdf_info, 0, sizeof(SbdfInfo));
         if (e == hipSuccess)
             e = hipMemset(g->ctrl, 0, sizeof(Ctrl));
+        if (e == hipSuccess)
+            e = hipMemset(g->stat, 0, kStatWords * sizeof(unsigned long long));
         if (e != hipSuccess)
             st = fail(TSDF_HIP_ERROR, std::string("hipMemset: ") + hipGetErrorString(e));
     }
     alloc((void**)&g->view_dev, 256 * sizeof(ViewPose));
     if (st == TSDF_OK) {
         hipError_t e = hipHostMalloc((void**)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&g->stat_host, kStatWords * sizeof(unsigned long long), hipHostMallocDefault);
         if (e == hipSuccess)
             e = hipHostMalloc((void**)&g->view_host, 256 * sizeof(ViewPose), hipHostMallocDefault);
         if (e == hipSuccess)
@@ -1532,6 +1580,7 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->sbdf_info);
     (void)hipFree(g->ptmap);
     (void)hipFree(g->ctrl);
+    (void)hipFree(g->stat);
     (void)hipFree(g->op_mask);
     (void)hipFree(g->ptmap_batch);
     if (g->batch_exec)
@@ -1549,6 +1598,8 @@ void tsdf_destroy(tsdf_grid* g)
     free(g->prof_kind);
     if (g->ctrl_host)
         (void)hipHostFree(g->ctrl_host);
+    if (g->stat_host)
+        (void)hipHostFree(g->stat_host);
     if (g->view_host)
         (void)hipHostFree(g->view_host);
     if (g->render_done)
@@ -1883,7 +1934,7 @@ void tsdf_default_tuning(tsdf_tuning* t)
         return;
     t->raycast_xcd_map = 1;
     t->raycast_superblocks = 0;
-    t->integrate_wg_per_cu = 4;
+    t->integrate_wg_per_cu = 6;
     t->batch_ops_per_pass = 16;
     t->batch_graph = 1;
 }
@@ -1943,20 +1994,27 @@ tsdf_status tsdf_stats(tsdf_grid* g, tsdf_stats_out* out, void* stream)
     DeviceScope ds(g->p.device);
     hipStream_t s = (hipStream_t)stream;
     HIPC(hipMemcpyAsync(g->ctrl_host, g->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(g->stat_host, g->stat, kStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const Ctrl& c = *g->ctrl_host;
+    unsigned long long tot[kStCount];
+    for (int k = 0; k < kStCount; ++k) {
+        tot[k] = 0;
+        for (int l = 0; l < kStatLanes; ++l)
+            tot[k] += g->stat_host[((size_t)k * kStatLanes + l) * kStatStride];
+    }
     out->n_bricks = c.alloc_top < g->p.n_bricks ? c.alloc_top : g->p.n_bricks;
     out->capacity = g->p.n_bricks;
     out->overflow = c.overflow;
     out->last_touched = g->epoch ? c.touched_count[g->epoch & 1u] : 0;
     out->total_touched = c.total_touched;
-    out->total_new = c.total_new;
-    out->total_updated = c.total_updated;
-    out->total_valid_px = c.total_valid;
-    out->ray_hits = c.ray_hits;
-    out->ray_ub = c.ray_ub;
-    out->ray_samples = c.ray_samples;
-    out->ray_probes = c.ray_probes;
+    out->total_new = tot[kStNew];
+    out->total_updated = tot[kStUpdated];
+    out->total_valid_px = tot[kStValid];
+    out->ray_hits = tot[kStRayHits];
+    out->ray_ub = tot[kStRayUb];
+    out->ray_samples = tot[kStRaySamples];
+    out->ray_probes = tot[kStRayProbes];
     return TSDF_OK;
 }
 

@@ -99,12 +99,7 @@ __global__ __launch_bounds__(kThreads) void k_rehash_insert(GridView g, unsigned
         const int sb = sb_index(c.x, c.y, c.z);
         if (sb >= 0 && !(atomicOr(&g.sb_bits[sb >> 5], 1u << (sb & 31)) & (1u << (sb & 31))))
             atomicAdd(&g.ctrl->sb_version, 1u);
-        atomicMin(&g.ctrl->bbox_min[0], c.x);
-        atomicMin(&g.ctrl->bbox_min[1], c.y);
-        atomicMin(&g.ctrl->bbox_min[2], c.z);
-        atomicMax(&g.ctrl->bbox_max[0], c.x);
-        atomicMax(&g.ctrl->bbox_max[1], c.y);
-        atomicMax(&g.ctrl->bbox_max[2], c.z);
+        grow_bbox(g.ctrl, c.x, c.y, c.z);
         if (j == 0)
             g.ctrl->alloc_top = n;
     }

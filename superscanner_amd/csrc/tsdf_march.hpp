@@ -22,18 +22,30 @@ struct Ctrl {
     unsigned int overflow;          // refused allocations / full table
     unsigned int touched_count[2];  // touched-list length, by call parity
     unsigned long long total_touched;
-    unsigned long long total_new;
-    unsigned long long total_updated;
-    unsigned long long total_valid;
-    unsigned long long ray_hits;
-    unsigned long long ray_ub;
-    unsigned long long ray_samples;
-    unsigned long long ray_probes;
     int bbox_min[3];  // bounding box of every block ever inserted since reset
     int bbox_max[3];
     unsigned int sb_version;  // bumped when a superblock bit is set or the bits are cleared
     unsigned int pad_;
 };
+
+// Statistics counters that many workgroups add to live outside Ctrl, each
+// spread over kStatLanes cache lines: a wavefront or workgroup adds its count
+// to one lane (its index mod kStatLanes) and the reader sums the lanes.
+// Device-scope atomics on one address from every XCD queue up behind each
+// other; spread over 32 lines they do not.
+enum StatId : int {
+    kStUpdated = 0,   // voxel updates (integrate, deintegrate, batch)
+    kStValid,         // pixels with z > rmin (collection)
+    kStNew,           // bricks allocated
+    kStRayHits,       // raycast: zero crossings found
+    kStRayUb,         // raycast: rays that ran into undefined behaviour
+    kStRaySamples,    // raycast: voxel samples
+    kStRayProbes,     // raycast: block lookups
+    kStCount
+};
+constexpr int kStatLanes = 32;
+constexpr int kStatStride = 16;  // unsigned long longs per lane: one 128-byte line
+constexpr size_t kStatWords = (size_t)kStCount * kStatLanes * kStatStride;
 
 // Empty-space distance over superblocks: for the superblocks inside the box
 // of the set superblock bits, the Chebyshev distance (in superblocks, capped
@@ -60,6 +72,7 @@ struct GridView {
     uint8_t* sbdf;              // [kSbdfCap] empty-space distance over superblocks
     SbdfInfo* sbdf_info;
     Ctrl* ctrl;
+    unsigned long long* stat;   // [kStCount][kStatLanes][kStatStride]
     uint32_t mask;              // n_buckets - 1
     uint32_t n_bricks;
     float vl, vbl, trunc, rmin, dmax, wmax;
