@@ -60,7 +60,7 @@ struct Offsets {
 
 constexpr int kThreads = 256;
 #ifndef TSDF_TILE_W
-#define TSDF_TILE_W 64
+#define TSDF_TILE_W 32
 #endif
 #ifndef TSDF_TILE_H
 #define TSDF_TILE_H 16
@@ -68,7 +68,7 @@ constexpr int kThreads = 256;
 #ifndef TSDF_LDS_SET
 #define TSDF_LDS_SET 2048
 #endif
-constexpr int kTileW = TSDF_TILE_W;   // collection tile: 64 x 16 pixels = 16 patches of 8 x 8
+constexpr int kTileW = TSDF_TILE_W;   // collection tile: 32 x 16 pixels = 8 patches of 8 x 8
 constexpr int kTileH = TSDF_TILE_H;
 constexpr int kLdsSet = TSDF_LDS_SET; // per-tile block set in LDS (2048 x 8 B = 16 KiB)
 constexpr int kMaxOps = 256;    // ops per batched pass
@@ -294,7 +294,7 @@ __global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
 // ---------------------------------------------------------------------------
 // k_collect: block allocation (vbCollectionTag, GPUVoxelGrid.cpp:471-514)
 // ---------------------------------------------------------------------------
-// One workgroup per 64 x 16 pixel tile.  Each valid pixel enumerates the
+// One workgroup per 32 x 16 pixel tile.  Each valid pixel enumerates the
 // blocks of its +-trunc box into an LDS set, so the global table sees each
 // block once per tile; then the tile's set is claimed against the global
 // table with wavefront-aggregated appends to the touched list.  The pixel's
@@ -312,7 +312,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
         s_valid = 0;
     __syncthreads();
 
-    // Each wavefront takes 8 x 8 pixel patches of the 64 x 16 tile (lane =
+    // Each wavefront takes 8 x 8 pixel patches of the 32 x 16 tile (lane =
     // pixel of the patch).  The blocks of a patch are the union of its
     // pixels' +-trunc boxes; when their common bounding box has at most 128
     // cells, that union is built exactly as a 128-bit mask over the box (each
@@ -441,7 +441,15 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
             uniq[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = v;
     }
     __syncthreads();
+    // The slots this tile must append are gathered in LDS (lset is free
+    // now) and appended with one global atomic for the whole tile: the
+    // touched-list counter is one address that every tile adds to.
     const unsigned int nu = n_uniq;
+    uint32_t* app = reinterpret_cast<uint32_t*>(lset);  // 2 * kLdsSet >= nu entries
+    __shared__ unsigned int n_app, app_base;
+    if (threadIdx.x == 0)
+        n_app = 0;
+    __syncthreads();
     for (unsigned int e = threadIdx.x; e < ((nu + 63u) & ~63u); e += kThreads) {
         bool need = false;
         uint32_t slot = 0;
@@ -450,8 +458,15 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
             need = claim<kBatch>(g, unpack21(v >> 42), unpack21(v >> 21), unpack21(v), epoch, op_mask, op_index,
                                  &slot);
         }
-        wave_append(need, slot, &g.ctrl->touched_count[par], g.touched);
+        wave_append(need, slot, &n_app, app);
     }
+    __syncthreads();
+    const unsigned int na = n_app;
+    if (threadIdx.x == 0 && na)
+        app_base = atomicAdd(&g.ctrl->touched_count[par], na);
+    __syncthreads();
+    for (unsigned int e = threadIdx.x; e < na; e += kThreads)
+        g.touched[app_base + e] = app[e];
 }
 
 __global__ __launch_bounds__(kThreads) void k_collect(GridView g, OpDesc op, uint32_t epoch, int par)
