@@ -70,6 +70,13 @@ constexpr int kThreads = 256;
 #endif
 constexpr int kTileW = TSDF_TILE_W;   // collection tile: 32 x 16 pixels = 8 patches of 8 x 8
 constexpr int kTileH = TSDF_TILE_H;
+// A batch pass collects several ops at once (grid.z = op), which fills the
+// GPU with larger tiles; larger tiles also see a block fewer times per op.
+#ifndef TSDF_BATCH_TILE_W
+#define TSDF_BATCH_TILE_W 64
+#endif
+constexpr int kBatchTileW = TSDF_BATCH_TILE_W;
+constexpr int kBatchTileH = 16;
 constexpr int kLdsSet = TSDF_LDS_SET; // per-tile block set in LDS (2048 x 8 B = 16 KiB)
 constexpr int kMaxOps = 256;    // ops per batched pass
 constexpr int kMaskWords = kMaxOps / 32;
@@ -203,11 +210,19 @@ __device__ __forceinline__ bool claim(const GridView& g, int x, int y, int z, ui
     const uint32_t b0 = mix32(key) & g.mask;
     const unsigned long long s0 = __hip_atomic_load(&g.slots[b0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t e0 = __hip_atomic_load(&g.slot_epoch[b0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // a batch op's bit, read with the home bucket: most (block, op) pairs are
+    // claimed by several tiles, and only the first needs the atomic
+    const uint32_t bit = 1u << (op & 31);
+    const uint32_t m0 =
+        kBatch ? __hip_atomic_load(&op_mask[(size_t)b0 * kMaskWords + (op >> 5)], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT)
+               : 0u;
     int s;
-    uint32_t pre;
+    uint32_t pre, mword;
     if (s0 != kSlotEmpty && (uint32_t)(s0 >> 32) == key) {
         s = (int)b0;
         pre = e0;
+        mword = m0;
     } else {
         s = find_or_insert(g, key, x, y, z);
         if (s < 0) {
@@ -215,9 +230,11 @@ __device__ __forceinline__ bool claim(const GridView& g, int x, int y, int z, ui
             return false;
         }
         pre = __hip_atomic_load(&g.slot_epoch[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mword = 0u;
     }
-    if (kBatch)
-        atomicOr(&op_mask[(size_t)s * kMaskWords + (op >> 5)], 1u << (op & 31));
+    // bits are only set during collection, so one seen set stays set
+    if (kBatch && !(mword & bit))
+        atomicOr(&op_mask[(size_t)s * kMaskWords + (op >> 5)], bit);
     *slot_out = (uint32_t)s;
     if (pre == epoch)
         return false;  // another tile of this call claimed it already
@@ -300,7 +317,7 @@ __global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
 // table with wavefront-aggregated appends to the touched list.  The pixel's
 // world point goes to ptmap for k_integrate (the reference's inputPcd,
 // GPUVoxelGrid.h:303-307, transformed as at GPUVoxelGrid.cpp:356).
-template <bool kBatch>
+template <bool kBatch, int kTW, int kTH>
 __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch, int par,
                              uint32_t* op_mask, int op_index)
 {
@@ -321,9 +338,9 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
     // fall back to per-pixel enumeration.
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
     unsigned int nvalid = 0;
-    for (int pt_idx = wv; pt_idx < (kTileW / 8) * (kTileH / 8); pt_idx += kThreads / 64) {
-        const int i = blockIdx.y * kTileH + (pt_idx / (kTileW / 8)) * 8 + (lane >> 3);
-        const int j = blockIdx.x * kTileW + (pt_idx % (kTileW / 8)) * 8 + (lane & 7);
+    for (int pt_idx = wv; pt_idx < (kTW / 8) * (kTH / 8); pt_idx += kThreads / 64) {
+        const int i = blockIdx.y * kTH + (pt_idx / (kTW / 8)) * 8 + (lane >> 3);
+        const int j = blockIdx.x * kTW + (pt_idx % (kTW / 8)) * 8 + (lane & 7);
         const bool in = i < g.H && j < g.W;
         int x0 = 0x7FFFFFFF, y0 = 0x7FFFFFFF, z0 = 0x7FFFFFFF;
         int x1 = (int)0x80000000u, y1 = (int)0x80000000u, z1 = (int)0x80000000u;
@@ -471,7 +488,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
 
 __global__ __launch_bounds__(kThreads) void k_collect(GridView g, OpDesc op, uint32_t epoch, int par)
 {
-    collect_tile<false>(g, op, epoch, par, nullptr, 0);
+    collect_tile<false, kTileW, kTileH>(g, op, epoch, par, nullptr, 0);
 }
 
 // A batch pass reads its call stamp from device memory (*epoch_base + pass)
@@ -481,7 +498,7 @@ __global__ __launch_bounds__(kThreads) void k_collect_batch(GridView g, const Op
                                                             const uint32_t* epoch_base, int pass, uint32_t* op_mask)
 {
     const uint32_t epoch = *epoch_base + (uint32_t)pass;
-    collect_tile<true>(g, ops[blockIdx.z], epoch, (int)(epoch & 1u), op_mask, blockIdx.z);
+    collect_tile<true, kBatchTileW, kBatchTileH>(g, ops[blockIdx.z], epoch, (int)(epoch & 1u), op_mask, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1715,7 +1732,7 @@ void enqueue_batch_passes(tsdf_grid* g, int n_ops, int per, hipStream_t s, bool 
     const uint32_t* ep = &g->stage_dev->epoch_base;
     for (int cb = 0, pass = 0; cb < n_ops; cb += per, ++pass) {
         const int n = (n_ops - cb) < per ? (n_ops - cb) : per;
-        const dim3 cgrid((g->p.width + kTileW - 1) / kTileW, (g->p.height + kTileH - 1) / kTileH, n);
+        const dim3 cgrid((g->p.width + kBatchTileW - 1) / kBatchTileW, (g->p.height + kBatchTileH - 1) / kBatchTileH, n);
         const OpDesc* ops = g->stage_dev->ops + cb;
         {
             ProfScope ps(g, s, TSDF_K_COLLECT_BATCH, profile);
