@@ -63,13 +63,18 @@ constexpr int kThreads = 256;
 #define TSDF_TILE_W 32
 #endif
 #ifndef TSDF_TILE_H
-#define TSDF_TILE_H 16
+#define TSDF_TILE_H 8
 #endif
 #ifndef TSDF_LDS_SET
-#define TSDF_LDS_SET 2048
+#define TSDF_LDS_SET 1024
 #endif
-constexpr int kTileW = TSDF_TILE_W;   // collection tile: 32 x 16 pixels = 8 patches of 8 x 8
+// Collection tile: 32 x 8 pixels = 4 patches of 8 x 8, one per wavefront.
+// Its block set holds kLdsSet blocks (8 KiB); a tile sees about 40 blocks on
+// average, but grazing views put hundreds in one tile, and a full set falls
+// back to claiming blocks one by one, which costs far more than the set.
+constexpr int kTileW = TSDF_TILE_W;
 constexpr int kTileH = TSDF_TILE_H;
+constexpr int kLdsSet = TSDF_LDS_SET;
 // A batch pass collects several ops at once (grid.z = op), which fills the
 // GPU with larger tiles; larger tiles also see a block fewer times per op.
 #ifndef TSDF_BATCH_TILE_W
@@ -77,7 +82,7 @@ constexpr int kTileH = TSDF_TILE_H;
 #endif
 constexpr int kBatchTileW = TSDF_BATCH_TILE_W;
 constexpr int kBatchTileH = 16;
-constexpr int kLdsSet = TSDF_LDS_SET; // per-tile block set in LDS (2048 x 8 B = 16 KiB)
+constexpr int kBatchLdsSet = 2048;  // four times the plain tile's pixels
 constexpr int kMaxOps = 256;    // ops per batched pass
 constexpr int kMaskWords = kMaxOps / 32;
 
@@ -258,21 +263,22 @@ __device__ __forceinline__ void set_bits128(unsigned long long& lo, unsigned lon
         hi |= bit_range(base > 64 ? base - 64 : 0, e - 64);
 }
 
+template <int kSet>
 __device__ __forceinline__ bool lds_insert(unsigned long long* set, unsigned long long v)
 {
-    uint32_t h = mix32((uint32_t)v ^ ((uint32_t)(v >> 32) * 0x9E3779B1u)) & (kLdsSet - 1);
+    uint32_t h = mix32((uint32_t)v ^ ((uint32_t)(v >> 32) * 0x9E3779B1u)) & (kSet - 1);
     for (int p = 0; p < 32; ++p) {
         const unsigned long long cur = set[h];  // most candidates are already present
         if (cur == v)
             return true;
         if (cur != kSlotEmpty) {
-            h = (h + 1) & (kLdsSet - 1);
+            h = (h + 1) & (kSet - 1);
             continue;
         }
         const unsigned long long old = atomicCAS(&set[h], kSlotEmpty, v);
         if (old == kSlotEmpty || old == v)
             return true;
-        h = (h + 1) & (kLdsSet - 1);
+        h = (h + 1) & (kSet - 1);
     }
     return false;
 }
@@ -311,25 +317,25 @@ __global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
 // ---------------------------------------------------------------------------
 // k_collect: block allocation (vbCollectionTag, GPUVoxelGrid.cpp:471-514)
 // ---------------------------------------------------------------------------
-// One workgroup per 32 x 16 pixel tile.  Each valid pixel enumerates the
+// One workgroup per 32 x 8 pixel tile.  Each valid pixel enumerates the
 // blocks of its +-trunc box into an LDS set, so the global table sees each
 // block once per tile; then the tile's set is claimed against the global
 // table with wavefront-aggregated appends to the touched list.  The pixel's
 // world point goes to ptmap for k_integrate (the reference's inputPcd,
 // GPUVoxelGrid.h:303-307, transformed as at GPUVoxelGrid.cpp:356).
-template <bool kBatch, int kTW, int kTH>
+template <bool kBatch, int kTW, int kTH, int kSet>
 __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch, int par,
                              uint32_t* op_mask, int op_index)
 {
-    __shared__ unsigned long long lset[kLdsSet];
+    __shared__ unsigned long long lset[kSet];
     __shared__ unsigned int s_valid;
-    for (int e = threadIdx.x; e < kLdsSet; e += kThreads)
+    for (int e = threadIdx.x; e < kSet; e += kThreads)
         lset[e] = kSlotEmpty;
     if (threadIdx.x == 0)
         s_valid = 0;
     __syncthreads();
 
-    // Each wavefront takes 8 x 8 pixel patches of the 32 x 16 tile (lane =
+    // Each wavefront takes 8 x 8 pixel patches of the 32 x 8 tile (lane =
     // pixel of the patch).  The blocks of a patch are the union of its
     // pixels' +-trunc boxes; when their common bounding box has at most 128
     // cells, that union is built exactly as a 128-bit mask over the box (each
@@ -413,7 +419,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
                 const int cz = c / nxy, rem = c - cz * nxy;
                 const int cy = rem / nx, cx = rem - cy * nx;
                 const int x = ux0 + cx, y = uy0 + cy, z = uz0 + cz;
-                if (!lds_insert(lset, pack_coord(x, y, z))) {  // set full: claim directly
+                if (!lds_insert<kSet>(lset, pack_coord(x, y, z))) {  // set full: claim directly
                     uint32_t slot;
                     if (claim<kBatch>(g, x, y, z, epoch, op_mask, op_index, &slot))
                         g.touched[atomicAdd(&g.ctrl->touched_count[par], 1u)] = slot;
@@ -423,7 +429,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
             for (int x = x0; x <= x1; ++x)
                 for (int y = y0; y <= y1; ++y)
                     for (int z = z0; z <= z1; ++z) {
-                        if (!lds_insert(lset, pack_coord(x, y, z))) {  // set full: claim directly
+                        if (!lds_insert<kSet>(lset, pack_coord(x, y, z))) {  // set full: claim directly
                             uint32_t slot;
                             if (claim<kBatch>(g, x, y, z, epoch, op_mask, op_index, &slot))
                                 g.touched[atomicAdd(&g.ctrl->touched_count[par], 1u)] = slot;
@@ -441,12 +447,12 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
     // compact the tile's distinct blocks, then claim them all in one round
     // (one dependent chain of global accesses per tile rather than one per
     // stripe of the set)
-    __shared__ unsigned long long uniq[kLdsSet];
+    __shared__ unsigned long long uniq[kSet];
     __shared__ unsigned int n_uniq;
     if (threadIdx.x == 0)
         n_uniq = 0;
     __syncthreads();
-    for (int e = threadIdx.x; e < kLdsSet; e += kThreads) {
+    for (int e = threadIdx.x; e < kSet; e += kThreads) {
         const unsigned long long v = lset[e];
         const bool has = v != kSlotEmpty;
         const unsigned long long m = __ballot(has);
@@ -462,7 +468,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
     // now) and appended with one global atomic for the whole tile: the
     // touched-list counter is one address that every tile adds to.
     const unsigned int nu = n_uniq;
-    uint32_t* app = reinterpret_cast<uint32_t*>(lset);  // 2 * kLdsSet >= nu entries
+    uint32_t* app = reinterpret_cast<uint32_t*>(lset);  // 2 * kSet >= nu entries
     __shared__ unsigned int n_app, app_base;
     if (threadIdx.x == 0)
         n_app = 0;
@@ -488,7 +494,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
 
 __global__ __launch_bounds__(kThreads) void k_collect(GridView g, OpDesc op, uint32_t epoch, int par)
 {
-    collect_tile<false, kTileW, kTileH>(g, op, epoch, par, nullptr, 0);
+    collect_tile<false, kTileW, kTileH, kLdsSet>(g, op, epoch, par, nullptr, 0);
 }
 
 // A batch pass reads its call stamp from device memory (*epoch_base + pass)
@@ -498,7 +504,8 @@ __global__ __launch_bounds__(kThreads) void k_collect_batch(GridView g, const Op
                                                             const uint32_t* epoch_base, int pass, uint32_t* op_mask)
 {
     const uint32_t epoch = *epoch_base + (uint32_t)pass;
-    collect_tile<true, kBatchTileW, kBatchTileH>(g, ops[blockIdx.z], epoch, (int)(epoch & 1u), op_mask, blockIdx.z);
+    collect_tile<true, kBatchTileW, kBatchTileH, kBatchLdsSet>(g, ops[blockIdx.z], epoch, (int)(epoch & 1u), op_mask,
+                                                               blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
