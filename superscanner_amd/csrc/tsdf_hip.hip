@@ -651,7 +651,22 @@ __device__ __forceinline__ void load_lane(const GridView& g, int id, int lane, b
 }
 
 // Writes back the voxels flagged in `dirty` (all of them for a fresh brick,
-// whose payload memory was never initialised).
+// whose payload memory was never initialised).  A 128-byte line of a z
+// plane is written whole when any voxel in it changed (TSDF_STORE_LINES):
+// the other lanes of the line store the values they loaded, which is exact
+// because one wavefront owns the brick, and whole lines cost less HBM write
+// traffic than partial ones.  All lanes of the wavefront must call this.
+#ifndef TSDF_STORE_LINES
+#define TSDF_STORE_LINES 1
+#endif
+__device__ __forceinline__ bool line_dirty(bool mine, int lane, int lanes_per_line)
+{
+    const unsigned long long b = __ballot(mine);
+    const unsigned long long gm = lanes_per_line >= 64 ? ~0ull
+                                                       : (((1ull << lanes_per_line) - 1ull) << (lane & ~(lanes_per_line - 1)));
+    return (b & gm) != 0ull;
+}
+
 template <typename T, bool kColor>
 __device__ __forceinline__ void store_lane(const GridView& g, int id, int lane, unsigned int dirty,
                                            const LaneVoxels<kColor>& v)
@@ -660,11 +675,13 @@ __device__ __forceinline__ void store_lane(const GridView& g, int id, int lane, 
     float* cr = kColor ? g.rgb + (size_t)id * (3 * kVPB) : nullptr;
 #pragma unroll
     for (int z = 0; z < kVPE; ++z) {
-        if (!((dirty >> z) & 1u))
-            continue;
+        const bool mine = (dirty >> z) & 1u;
+        const bool wp = TSDF_STORE_LINES ? line_dirty(mine, lane, 128 / (2 * (int)sizeof(T))) : mine;
+        const bool wc = kColor && (TSDF_STORE_LINES ? line_dirty(mine, lane, 32) : mine);
         const int k = lane + 64 * z;
-        st_pair(sd, k, v.s[z], v.w[z]);
-        if (kColor) {
+        if (wp)
+            st_pair(sd, k, v.s[z], v.w[z]);
+        if (wc) {
             cr[k] = v.c[z][0];
             cr[k + kVPB] = v.c[z][1];
             cr[k + 2 * kVPB] = v.c[z][2];
