@@ -741,7 +741,10 @@ __device__ __forceinline__ WaveRange wave_range(unsigned int count)
 // wavefronts update the chunk's bricks.  Within a brick, the eight payload
 // loads and the eight depth-point gathers of a lane are issued together.
 // ---------------------------------------------------------------------------
-constexpr int kMetaChunk = 16;
+#ifndef TSDF_META_CHUNK
+#define TSDF_META_CHUNK 8
+#endif
+constexpr int kMetaChunk = TSDF_META_CHUNK;  // touched entries resolved per workgroup round
 
 // Adds every thread's `v` to statistics counter `id` with one atomic per
 // workgroup, on the workgroup's lane (all threads of the workgroup must call
