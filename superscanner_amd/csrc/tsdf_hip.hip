@@ -1991,7 +1991,7 @@ void tsdf_default_tuning(tsdf_tuning* t)
 {
     if (!t)
         return;
-    t->raycast_xcd_map = 1;
+    t->raycast_xcd_map = 0;
     t->raycast_superblocks = 0;
     t->integrate_wg_per_cu = 6;
     t->batch_ops_per_pass = 16;
