@@ -113,6 +113,10 @@ typedef struct {
                                   /* batch, 1..256                              */
     int32_t batch_graph;          /* 1: a batch of <= 256 ops runs as one       */
                                   /* captured graph (staging copy + passes)     */
+    int32_t raycast_order;        /* order the raycast's tiles are dispatched   */
+                                  /* in (results do not depend on it): 0 the    */
+                                  /* hardware's, 1 reversed, 2 the tiles that   */
+                                  /* took longest in the previous raycast first */
 } tsdf_tuning;
 
 /* Camera of a render view (the differentiable render has its own resolution

@@ -12,6 +12,8 @@ The tunables are the fields of ``tsdf_tuning`` (include/tsdf.h):
   batch_graph           a batch of <= 256 ops as one captured graph
   raycast_xcd_map       XCD-banded tile order of the raycast
   raycast_superblocks   certified empty-superblock skipping in the raycast
+  raycast_order         dispatch order of the raycast's tiles (slowest of the
+                        previous raycast first, reversed, or the hardware's)
 
 Each tuner times candidates on the caller's own representative workload with
 HIP events on the grid's stream and returns the fastest setting together with
@@ -88,19 +90,20 @@ def tune_integrate(grid, depths, poses, *, wg_per_cu=INTEGRATE_WG_PER_CU):
     return {"integrate_wg_per_cu": wg}, table
 
 
-def tune_raycast(grid, poses, *, xcd_map=(0, 1), superblocks=(0, 1)):
-    """Picks raycast_xcd_map and raycast_superblocks for ``grid.raycast``.
-    Returns (best tuning dict, [(xcd_map, superblocks, ms), ...])."""
+def tune_raycast(grid, poses, *, xcd_map=(0, 1), superblocks=(0, 1), order=(0, 1, 2)):
+    """Picks raycast_xcd_map, raycast_superblocks and raycast_order for
+    ``grid.raycast``.  Returns (best tuning dict, [(xcd_map, superblocks,
+    order, ms), ...])."""
     table = []
-    for xm, sb in itertools.product(xcd_map, superblocks):
-        grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb)
+    for xm, sb, od in itertools.product(xcd_map, superblocks, order):
+        grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb, raycast_order=od)
 
         def run():
             for E in poses:
                 grid.raycast(E)
 
-        run()  # warm
-        table.append((xm, sb, _timed(run)))
-    xm, sb, _ = min(table, key=lambda r: r[2])
-    grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb)
-    return {"raycast_xcd_map": xm, "raycast_superblocks": sb}, table
+        run()  # warm (and, for order 2, the costs the first timed raycast orders by)
+        table.append((xm, sb, od, _timed(run)))
+    xm, sb, od, _ = min(table, key=lambda r: r[3])
+    grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb, raycast_order=od)
+    return {"raycast_xcd_map": xm, "raycast_superblocks": sb, "raycast_order": od}, table

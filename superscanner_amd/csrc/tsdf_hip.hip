@@ -1016,7 +1016,15 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     // workgroups of one XCD take one contiguous band of tiles, so the bricks
     // one XCD's L2 holds belong to neighbouring rays.
     const int per = (n_tiles + 7) >> 3;
-    const int tile = rp.xcd_map ? (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const int b = (int)blockIdx.x;
+    int tile = rp.xcd_map ? (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3) : b;
+    // the dispatch order (the results do not depend on it): the last
+    // workgroups dispatched start only when earlier ones finish, so a slow
+    // tile dispatched late adds its whole time to the launch
+    if (rp.order == 1 && b < n_tiles)
+        tile = n_tiles - 1 - b;
+    else if (rp.order == 2 && b < n_tiles)
+        tile = (int)rp.tile_order[b];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = (tile / tiles_x) * kRayTileH + (wv / kRayWx) * 8 + (lane >> 3);
     const int j = (tile % tiles_x) * kRayTileW + (wv % kRayWx) * 8 + (lane & 7);
@@ -1075,6 +1083,15 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     bad = wave_sum(bad);
     samples = wave_sum(samples);
     probes = wave_sum(probes);
+    // this wavefront's work for the next raycast's schedule: its longest
+    // ray's samples + lookups
+    if (!kStats && rp.tile_cost && tile < n_tiles) {
+        unsigned int c = samples + probes;
+        for (int o = 32; o > 0; o >>= 1)
+            c = max(c, (unsigned int)__shfl_xor((int)c, o));
+        if (lane == 0)
+            rp.tile_cost[(size_t)tile * kRayWaves + wv] = c;
+    }
     // one atomic per wavefront and counter, on the wavefront's lane: the
     // wavefronts end at very different times, so none waits for the others
     if (lane == 0) {
@@ -1083,6 +1100,56 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
         if (bad) atomicAdd(stat_lane(g, kStRayUb, who), (unsigned long long)bad);
         if (samples) atomicAdd(stat_lane(g, kStRaySamples, who), (unsigned long long)samples);
         if (probes) atomicAdd(stat_lane(g, kStRayProbes, who), (unsigned long long)probes);
+    }
+}
+
+// The slowest-first tile order for the next raycast: tiles sorted by the
+// work of their slowest wavefront in this one, largest first (a counting
+// sort over kOrderBuckets cost levels; the order within a level is
+// arbitrary).  One workgroup.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderBuckets = 256;
+__global__ __launch_bounds__(kOrderThreads) void k_ray_order(const uint32_t* cost, int n_tiles, uint32_t* order)
+{
+    __shared__ unsigned int hist[kOrderBuckets];
+    __shared__ unsigned int cmax;
+    for (int k = threadIdx.x; k < kOrderBuckets; k += kOrderThreads)
+        hist[k] = 0;
+    if (threadIdx.x == 0)
+        cmax = 0;
+    __syncthreads();
+    unsigned int m = 0;
+    for (int t = threadIdx.x; t < n_tiles; t += kOrderThreads) {
+        unsigned int c = 0;
+        for (int w = 0; w < kRayWaves; ++w)
+            c = max(c, cost[(size_t)t * kRayWaves + w]);
+        m = max(m, c);
+    }
+    atomicMax(&cmax, m);
+    __syncthreads();
+    const unsigned long long top = (unsigned long long)cmax + 1ull;
+    // level kOrderBuckets - 1 holds the slowest tiles
+    for (int t = threadIdx.x; t < n_tiles; t += kOrderThreads) {
+        unsigned int c = 0;
+        for (int w = 0; w < kRayWaves; ++w)
+            c = max(c, cost[(size_t)t * kRayWaves + w]);
+        atomicAdd(&hist[(unsigned int)((unsigned long long)c * kOrderBuckets / top)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive offsets, slowest level first
+        unsigned int run = 0;
+        for (int k = kOrderBuckets - 1; k >= 0; --k) {
+            const unsigned int h = hist[k];
+            hist[k] = run;
+            run += h;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n_tiles; t += kOrderThreads) {
+        unsigned int c = 0;
+        for (int w = 0; w < kRayWaves; ++w)
+            c = max(c, cost[(size_t)t * kRayWaves + w]);
+        order[atomicAdd(&hist[(unsigned int)((unsigned long long)c * kOrderBuckets / top)], 1u)] = (uint32_t)t;
     }
 }
 
@@ -1157,6 +1224,9 @@ struct tsdf_grid {
     Ctrl* ctrl_host;  // pinned
     unsigned long long* stat;       // statistics counters, kStatWords
     unsigned long long* stat_host;  // pinned
+    uint32_t* ray_cost;             // [raycast tiles * waves] work of the last raycast's wavefronts
+    uint32_t* ray_order;            // [raycast tiles] slowest-first order built from it
+    bool ray_order_valid;
     // batch buffers (allocated on first batched call)
     uint32_t* op_mask;
     float4* ptmap_batch;
@@ -1424,6 +1494,11 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
     rp.H = g->p.height;
     rp.W = g->p.width;
     rp.xcd_map = g->tune.raycast_xcd_map;
+    rp.order = kStats ? 0 : g->tune.raycast_order;
+    rp.tile_order = g->ray_order;
+    rp.tile_cost = (!kStats && rp.order == 2) ? g->ray_cost : nullptr;
+    if (rp.order == 2 && !g->ray_order_valid)
+        rp.order = 0;  // no previous raycast to order by yet
     rp.use_sb = g->tune.raycast_superblocks;
     rp.df = nullptr;
     for (int a = 0; a < 3; ++a)
@@ -1452,6 +1527,10 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
         else
             hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, block, 0, s, view(g), P, rp, points,
                                colors, depth, ub, fp);
+    }
+    if (rp.tile_cost) {  // the next raycast's slowest-first order
+        hipLaunchKernelGGL(k_ray_order, dim3(1), dim3(kOrderThreads), 0, s, g->ray_cost, nt, g->ray_order);
+        g->ray_order_valid = true;
     }
 }
 
@@ -1585,6 +1664,11 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
     alloc((void**)&g->ptmap, (size_t)p->height * p->width * sizeof(float4));
     alloc((void**)&g->ctrl, sizeof(Ctrl));
     alloc((void**)&g->stat, kStatWords * sizeof(unsigned long long));
+    {
+        const size_t tiles = (size_t)((p->width + kRayTileW - 1) / kRayTileW) * ((p->height + kRayTileH - 1) / kRayTileH);
+        alloc((void**)&g->ray_cost, tiles * kRayWaves * sizeof(uint32_t));
+        alloc((void**)&g->ray_order, tiles * sizeof(uint32_t));
+    }
     if (st == TSDF_OK) {
         // no distance field yet; counters start from zero
         hipError_t e = hipMemset(g->sbdf_info, 0, sizeof(SbdfInfo));
@@ -1640,6 +1724,8 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->ptmap);
     (void)hipFree(g->ctrl);
     (void)hipFree(g->stat);
+    (void)hipFree(g->ray_cost);
+    (void)hipFree(g->ray_order);
     (void)hipFree(g->op_mask);
     (void)hipFree(g->ptmap_batch);
     if (g->batch_exec)
@@ -1996,6 +2082,7 @@ void tsdf_default_tuning(tsdf_tuning* t)
     t->integrate_wg_per_cu = 6;
     t->batch_ops_per_pass = 16;
     t->batch_graph = 1;
+    t->raycast_order = 2;
 }
 
 tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
@@ -2005,8 +2092,9 @@ tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
     if (t->raycast_xcd_map < 0 || t->raycast_xcd_map > 1 || t->raycast_superblocks < 0 ||
         t->raycast_superblocks > 1 || t->integrate_wg_per_cu < 1 ||
         t->integrate_wg_per_cu > 32 || t->batch_ops_per_pass < 1 || t->batch_ops_per_pass > kMaxOps ||
-        t->batch_graph < 0 || t->batch_graph > 1)
-        return fail(TSDF_BAD_ARG, "workgroups per CU must lie in [1, 32]");
+        t->batch_graph < 0 || t->batch_graph > 1 || t->raycast_order < 0 || t->raycast_order > 2)
+        return fail(TSDF_BAD_ARG, "tuning field out of range (workgroups per CU in [1, 32], raycast order 0, 1 "
+                                  "or 2, flags 0 or 1)");
     g->tune = *t;
     return TSDF_OK;
 }

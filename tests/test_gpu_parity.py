@@ -84,6 +84,20 @@ def test_plain_small_full_size(gpu):
     fp = g.raycast_footprint(poses[7])
     r = o.raycast(poses[7], stats=True)["counts"]
     assert fp["u_vox"] == r["u_vox"] and fp["u_blk"] == r["u_blk"], (fp, r)
+    # the dispatch order of the raycast's tiles (reversed; slowest of the
+    # previous raycast first) changes no point and no count
+    for order in (1, 2, 0):
+        g.set_tuning(raycast_order=order)
+        before = g.stats()
+        samples = probes = 0
+        for k in (22, 7, 15):
+            r = assert_raycast_equal(g, o, poses[k])
+            samples += r["counts"]["samples"]
+            probes += r["counts"]["probes"]
+        after = g.stats()
+        assert after["ray_samples"] - before["ray_samples"] == samples, order
+        assert after["ray_probes"] - before["ray_probes"] == probes, order
+    g.set_tuning(raycast_order=2)
     # the empty-space replay keeps every point and count
     g.set_tuning(raycast_superblocks=1)
     before = g.stats()
