@@ -1010,6 +1010,9 @@ __attribute__((amdgpu_waves_per_eu(TSDF_RAY_WAVES_PER_EU, TSDF_RAY_WAVES_PER_EU)
 void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors, float* depth, uint8_t* ub,
                Footprint fp)
 {
+    // the wavefront's start on the 100 MHz constant clock (its duration
+    // orders the next raycast's tiles)
+    const unsigned long long t_begin = __builtin_amdgcn_s_memrealtime();
     const int tiles_x = (rp.W + kRayTileW - 1) / kRayTileW;
     const int n_tiles = tiles_x * ((rp.H + kRayTileH - 1) / kRayTileH);
     // Workgroups are dealt to the 8 XCDs round-robin; with xcd_map the
@@ -1083,15 +1086,10 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     bad = wave_sum(bad);
     samples = wave_sum(samples);
     probes = wave_sum(probes);
-    // this wavefront's work for the next raycast's schedule: its longest
-    // ray's samples + lookups
-    if (!kStats && rp.tile_cost && tile < n_tiles) {
-        unsigned int c = samples + probes;
-        for (int o = 32; o > 0; o >>= 1)
-            c = max(c, (unsigned int)__shfl_xor((int)c, o));
-        if (lane == 0)
-            rp.tile_cost[(size_t)tile * kRayWaves + wv] = c;
-    }
+    // this wavefront's time, for the next raycast's schedule (it decides
+    // only the order of the tiles, never a result)
+    if (!kStats && rp.tile_cost && tile < n_tiles && lane == 0)
+        rp.tile_cost[(size_t)tile * kRayWaves + wv] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_begin);
     // one atomic per wavefront and counter, on the wavefront's lane: the
     // wavefronts end at very different times, so none waits for the others
     if (lane == 0) {
@@ -1104,9 +1102,9 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
 }
 
 // The slowest-first tile order for the next raycast: tiles sorted by the
-// work of their slowest wavefront in this one, largest first (a counting
-// sort over kOrderBuckets cost levels; the order within a level is
-// arbitrary).  One workgroup.
+// time of their slowest wavefront in this one, largest first (a counting
+// sort over kOrderBuckets levels; the order within a level is arbitrary).
+// One workgroup.
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderBuckets = 256;
 __global__ __launch_bounds__(kOrderThreads) void k_ray_order(const uint32_t* cost, int n_tiles, uint32_t* order)

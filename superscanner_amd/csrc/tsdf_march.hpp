@@ -252,7 +252,7 @@ struct RayParams {
     int xcd_map;      // 1: each XCD takes one band of the image
     int order;        // tile order: 0 hardware (or xcd_map), 1 reversed, 2 `tile_order`
     const uint32_t* tile_order;  // [tiles] the order of the slowest-first schedule
-    uint32_t* tile_cost;         // [tiles * waves] work of each wavefront, for the next schedule
+    uint32_t* tile_cost;         // [tiles * waves] time of each wavefront, for the next schedule
     int use_sb;       // empty-space replay allowed (alias certificate and distance field hold)
     const uint8_t* df;  // the distance field and its box (filled in by the kernel)
     int df_lo[3], df_dim[3];
