@@ -1134,13 +1134,29 @@ __global__ __launch_bounds__(kOrderThreads) void k_ray_order(const uint32_t* cos
         atomicAdd(&hist[(unsigned int)((unsigned long long)c * kOrderBuckets / top)], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {  // exclusive offsets, slowest level first
-        unsigned int run = 0;
-        for (int k = kOrderBuckets - 1; k >= 0; --k) {
-            const unsigned int h = hist[k];
-            hist[k] = run;
-            run += h;
+    // exclusive offsets, slowest level first: a scan over the levels, one
+    // per thread of the first kOrderBuckets threads (wavefront scans, then
+    // the totals of the wavefronts before)
+    __shared__ unsigned int wsum[kOrderBuckets / 64];
+    unsigned int h = 0, incl = 0;
+    const int k = kOrderBuckets - 1 - (int)threadIdx.x;
+    if ((int)threadIdx.x < kOrderBuckets) {
+        h = hist[k];
+        incl = h;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned int y = __shfl_up(incl, o);
+            if ((int)__lane_id() >= o)
+                incl += y;
         }
+        if (__lane_id() == 63)
+            wsum[threadIdx.x >> 6] = incl;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < kOrderBuckets) {
+        unsigned int pre = 0;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w)
+            pre += wsum[w];
+        hist[k] = pre + incl - h;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < n_tiles; t += kOrderThreads) {
