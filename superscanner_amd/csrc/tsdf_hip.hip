@@ -1107,6 +1107,10 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
 // One workgroup.
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderBuckets = 256;
+#ifndef TSDF_ORDER_EVERY
+#define TSDF_ORDER_EVERY 4
+#endif
+constexpr int kOrderEvery = TSDF_ORDER_EVERY;  // raycasts per rebuild of the order
 __global__ __launch_bounds__(kOrderThreads) void k_ray_order(const uint32_t* cost, int n_tiles, uint32_t* order)
 {
     __shared__ unsigned int hist[kOrderBuckets];
@@ -1241,6 +1245,7 @@ struct tsdf_grid {
     uint32_t* ray_cost;             // [raycast tiles * waves] work of the last raycast's wavefronts
     uint32_t* ray_order;            // [raycast tiles] slowest-first order built from it
     bool ray_order_valid;
+    int ray_order_age;              // raycasts since the order was rebuilt
     // batch buffers (allocated on first batched call)
     uint32_t* op_mask;
     float4* ptmap_batch;
@@ -1542,9 +1547,14 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
             hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, block, 0, s, view(g), P, rp, points,
                                colors, depth, ub, fp);
     }
-    if (rp.tile_cost) {  // the next raycast's slowest-first order
+    // the slowest-first order, rebuilt every kOrderEvery raycasts from the
+    // last one's times (consecutive poses see nearly the same rays)
+    if (rp.tile_cost && (!g->ray_order_valid || g->ray_order_age + 1 >= kOrderEvery)) {
         hipLaunchKernelGGL(k_ray_order, dim3(1), dim3(kOrderThreads), 0, s, g->ray_cost, nt, g->ray_order);
         g->ray_order_valid = true;
+        g->ray_order_age = 0;
+    } else if (rp.tile_cost) {
+        ++g->ray_order_age;
     }
 }
 
