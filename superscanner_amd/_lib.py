@@ -124,6 +124,14 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         raise ImportError(
             f"{path} is missing: build it with `make -C {_HERE}` "
             "(or __graft_entry__.build()); there is no CPU fallback")
+    # A process must hold one HIP runtime.  torch ships its own; loaded
+    # first, it is the one the library's HIP dependency resolves to, whereas
+    # loading the library first binds the system runtime and a later torch
+    # brings a second (hipSetDevice then fails in one of them).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     P = C.POINTER
     vp = C.c_void_p
