@@ -2104,7 +2104,7 @@ void tsdf_default_tuning(tsdf_tuning* t)
     t->raycast_xcd_map = 0;
     t->raycast_superblocks = 0;
     t->integrate_wg_per_cu = 6;
-    t->batch_ops_per_pass = 16;
+    t->batch_ops_per_pass = 8;
     t->batch_graph = 1;
     t->raycast_order = 2;
 }
