@@ -240,6 +240,12 @@ class GPUVoxelGrid:
         check(self._L.tsdf_get_tuning(self._h, C.byref(t)), "tsdf_get_tuning")
         return t.as_dict()
 
+    def get_tuning(self) -> dict:
+        """The grid's launch shapes (tsdf_tuning) as a dict."""
+        t = _lib.TsdfTuning()
+        check(self._L.tsdf_get_tuning(self._h, C.byref(t)), "tsdf_get_tuning")
+        return t.as_dict()
+
     def set_tuning(self, **kw):
         t = _lib.TsdfTuning()
         check(self._L.tsdf_get_tuning(self._h, C.byref(t)), "tsdf_get_tuning")

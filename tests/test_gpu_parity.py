@@ -283,6 +283,14 @@ def test_argument_errors(gpu):
     with pytest.raises(_lib.TsdfError) as e:
         h.integrate(None, d, np.eye(4, dtype=np.float32))
     assert e.value.status == _lib.TSDF_NO_INTRINSICS
+    # launch shapes out of range are refused and leave the tuning unchanged
+    before = g.get_tuning()
+    for bad_field in ({"raycast_order": 3}, {"raycast_order": -1}, {"integrate_wg_per_cu": 0},
+                      {"batch_ops_per_pass": 257}, {"raycast_xcd_map": 2}):
+        with pytest.raises(_lib.TsdfError) as e:
+            g.set_tuning(**bad_field)
+        assert e.value.status == _lib.TSDF_BAD_ARG, bad_field
+    assert g.get_tuning() == before
 
 
 def test_fp16_storage_error_bound(gpu):
