@@ -15,6 +15,8 @@
 // reference sums on the host in pixel order (and the RMSE in float).
 #pragma once
 
+#include <mutex>
+
 namespace {
 
 constexpr int kIcpMoments = TSDF_ICP_MOMENTS;
@@ -384,6 +386,46 @@ bool icp_params_ok(const tsdf_icp_params* p)
            p->n_search <= 64 && p->n_iterations >= 0;
 }
 
+// kokkosICP's workspace (the point and normal maps of both frames and the
+// association, the partial sums, a pinned copy of the final sums), kept per
+// device until the process ends and grown when a larger image comes: a
+// hipFree in every call would wait for the whole device.  Calls on one device
+// hold its mutex for their duration.
+struct IcpWorkspace {
+    std::mutex mu;
+    size_t n_cap = 0;  // pixels the point maps hold
+    float* buf = nullptr;
+    double* mom = nullptr;
+    double* mom_host = nullptr;
+};
+constexpr int kIcpMaxDevices = 64;
+IcpWorkspace g_icp_ws[kIcpMaxDevices];
+
+// Makes `w` hold n pixels (the caller holds w.mu); false on allocation failure.
+bool icp_workspace(IcpWorkspace& w, size_t n)
+{
+    if (!w.mom && hipMalloc(&w.mom, (size_t)(kIcpBlocks + 1) * kIcpMoments * sizeof(double)) != hipSuccess) {
+        w.mom = nullptr;
+        return false;
+    }
+    if (!w.mom_host && hipHostMalloc(&w.mom_host, kIcpMoments * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        w.mom_host = nullptr;
+        return false;
+    }
+    if (n > w.n_cap) {
+        if (w.buf)
+            (void)hipFree(w.buf);
+        w.buf = nullptr;
+        w.n_cap = 0;
+        if (hipMalloc(&w.buf, 4 * n * 3 * sizeof(float)) != hipSuccess) {
+            w.buf = nullptr;
+            return false;
+        }
+        w.n_cap = n;
+    }
+    return true;
+}
+
 // 4x4 inverse by cofactors (double, row-major); false when singular.
 bool inv4(const double* m, double* out)
 {
@@ -581,16 +623,17 @@ tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const doubl
     mat4_mul(tyi, tx, T);  // Txy = ty^-1 tx, kokkosICP.cpp:52
     const size_t n = (size_t)cam->height * cam->width;
     hipStream_t s = (hipStream_t)stream;
-    float* buf = nullptr;
-    double* mom = nullptr;
-    double* mom_host = nullptr;
-    HIPC(hipMalloc(&buf, 4 * n * 3 * sizeof(float)));
-    if (hipMalloc(&mom, (size_t)(kIcpBlocks + 1) * kIcpMoments * sizeof(double)) != hipSuccess ||
-        hipHostMalloc(&mom_host, kIcpMoments * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-        (void)hipFree(buf);
-        (void)hipFree(mom);
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kIcpMaxDevices)
+        return fail(TSDF_BAD_ARG, "device index beyond the ICP workspace table");
+    IcpWorkspace& ws = g_icp_ws[dev];
+    std::lock_guard<std::mutex> lock(ws.mu);
+    if (!icp_workspace(ws, n))
         return fail(TSDF_HIP_ERROR, "icp workspace allocation failed");
-    }
+    float* buf = ws.buf;
+    double* mom = ws.mom;
+    double* mom_host = ws.mom_host;
     float *x = buf, *y = buf + 3 * n, *yn = buf + 6 * n, *xn = buf + 9 * n;
     tsdf_status r = tsdf_depth_to_points(depth1, cam, x, stream);
     if (r == TSDF_OK)
@@ -646,9 +689,6 @@ tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const doubl
                 best[k] = T[k];
         }
     }
-    (void)hipFree(buf);
-    (void)hipFree(mom);
-    (void)hipHostFree(mom_host);
     if (r != TSDF_OK)
         return r;
     for (int k = 0; k < 16; ++k)
