@@ -65,7 +65,9 @@ typedef struct {
                             /* GPUVoxelGrid.cpp:11,27)                               */
     int32_t storage;        /* TSDF_STORAGE_*                                        */
     int32_t with_color;     /* keep a running-average rgb per voxel                  */
-    float max_weight;       /* 0 = uncapped, as the reference (GPUVoxelGrid.h:62)    */
+    float max_weight;       /* 0 = uncapped, as the reference (GPUVoxelGrid.h:62);   */
+                            /* fp16 storage caps at 2048 (refuses more), the        */
+                            /* largest weight fp16 counts exactly                   */
     float min_depth;        /* rmin, 0.2 m (GPUVoxelGrid.h:56)                       */
     float max_ray_length;   /* d_max, 10 m (GPUVoxelGrid.h:55)                       */
     int32_t device;         /* HIP device ordinal                                    */
@@ -134,6 +136,11 @@ void tsdf_default_params(tsdf_params *p);
  * GPUVoxelGrid.cpp:10-30.  Allocates the table and pool on p->device. */
 tsdf_status tsdf_create(const tsdf_params *p, tsdf_grid **out);
 
+/* The parameters the grid runs with (the ones given to tsdf_create, with
+ * the fp16 weight cap filled in); the reference's public members
+ * d_trunc, voxlength, resi, resj, maxVoxelBlocks (GPUVoxelGrid.h:50-62). */
+tsdf_status tsdf_get_params(const tsdf_grid *g, tsdf_params *p);
+
 /* Releases everything the handle owns (reference: destructor of the Views). */
 void tsdf_destroy(tsdf_grid *g);
 
@@ -149,8 +156,9 @@ tsdf_status tsdf_set_intrinsics(tsdf_grid *g, float fx, float fy, float cx, floa
 /* GPUVoxelGrid::integrate (sign = +1) and GPUVoxelGrid::deIntegrate
  * (sign = -1), GPUVoxelGrid.cpp:41-83 / 95-136, the (space, rgb, depth,
  * extr, withColor) overloads.  depth: device [H][pitch] uint16 mm; rgb:
- * device [H][W][3] uint8 or NULL (colour is integrated only when the grid was
- * created with with_color and rgb != NULL, the reference's withColor). */
+ * device [H][W][3] uint8, or NULL to update sdf and weight only (the
+ * reference's per-call withColor, GPUVoxelGrid.h:99-105); a grid created
+ * without colour storage refuses a non-NULL rgb with TSDF_BAD_ARG. */
 tsdf_status tsdf_integrate(tsdf_grid *g, const uint16_t *depth, int32_t pitch,
                            const uint8_t *rgb, const float E_cam2world[16],
                            int32_t sign, void *stream);
@@ -160,7 +168,9 @@ tsdf_status tsdf_integrate(tsdf_grid *g, const uint16_t *depth, int32_t pitch,
  * tsdf_integrate for op 0, 1, ..., n_ops-1 in turn.  Replaces the serial
  * loop of integrate.cpp:108-125 and the unused reintegratePP
  * (postprocessing.cpp:31-113).  depths/rgbs: HOST arrays of n_ops DEVICE
- * pointers (rgbs may be NULL); E: n_ops*16 host floats; signs: n_ops. */
+ * pointers (rgbs may be NULL, or hold NULL for an op without colour; a grid
+ * without colour storage refuses a non-NULL rgbs); E: n_ops*16 host floats;
+ * signs: n_ops. */
 tsdf_status tsdf_integrate_batch(tsdf_grid *g, int32_t n_ops,
                                  const uint16_t *const *depths, int32_t pitch,
                                  const uint8_t *const *rgbs, const float *E,
@@ -350,6 +360,11 @@ tsdf_status tsdf_icp_moments(const float *x_points, const float *x_nearest, cons
 tsdf_status tsdf_icp(const uint16_t *depth1, const uint16_t *depth2, const double tx[16], const double ty[16],
                      const tsdf_view *cam, const tsdf_icp_params *p, double T_out[16], float *rmse_out,
                      float *fitness_out, void *stream);
+
+/* Frees the workspace tsdf_icp keeps on the current device between calls
+ * (its point maps grow to the largest frame seen; no reference counterpart).
+ * The next tsdf_icp on the device allocates it again. */
+tsdf_status tsdf_icp_release(void);
 
 /* ---- reintegration scheduler helpers (host only, no device work) ------- */
 

@@ -84,6 +84,8 @@ def lib() -> C.CDLL:
     L.orc_export.restype = None
     L.orc_clean.argtypes = [vp]
     L.orc_clean.restype = C.c_int64
+    L.orc_import_rgb.argtypes = [vp, vp]
+    L.orc_import_rgb.restype = None
     L.orc_set_mutation.argtypes = [C.c_int]
     L.orc_set_mutation.restype = None
     dp = P(C.c_double)
@@ -232,6 +234,14 @@ class OracleGrid:
     def clean(self) -> int:
         """cleanMemory: erases all-zero-weight bricks; returns the count."""
         return int(lib().orc_clean(self._h))
+
+    def import_rgb(self, rgb):
+        """Overwrites every brick's colours ([n, 512, 3] in export order);
+        a test hook that gives voxels known colours."""
+        assert self.with_color
+        a = np.ascontiguousarray(rgb, dtype=np.float32)
+        assert a.shape == (self.n_bricks(), 512, 3)
+        lib().orc_import_rgb(self._h, a.ctypes.data)
 
     def export(self):
         n = self.n_bricks()

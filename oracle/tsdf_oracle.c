@@ -315,6 +315,80 @@ static inline f3 input_pcd(const orc_grid *g, const uint16_t *depth, int pitch, 
     return get_point(g, i, j, (float)depth[(int64_t)i * pitch + j] / 1000.0f);
 }
 
+/* A band's distinct blocks in order of first occurrence (collection). */
+typedef struct {
+    i3 *v;
+    int64_t n, cap;
+} band_list;
+
+static void band_push(band_list *b, i3 c)
+{
+    if (b->n == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 4096;
+        b->v = (i3 *)realloc(b->v, (size_t)b->cap * sizeof(i3));
+    }
+    b->v[b->n++] = c;
+}
+
+/* Open-addressing set of block coordinates (per collection thread). */
+typedef struct {
+    uint64_t *k; /* packed coordinate + 1; 0 = empty */
+    uint64_t cap, n;
+} bset;
+
+static uint64_t bpack(i3 c)
+{
+    return (((uint64_t)(uint32_t)c.x & 0x1FFFFFu) << 42) | (((uint64_t)(uint32_t)c.y & 0x1FFFFFu) << 21) |
+           ((uint64_t)(uint32_t)c.z & 0x1FFFFFu);
+}
+
+static void bset_init(bset *s)
+{
+    s->cap = 1u << 14;
+    s->n = 0;
+    s->k = (uint64_t *)calloc(s->cap, sizeof(uint64_t));
+}
+
+static void bset_free(bset *s) { free(s->k); }
+
+static uint64_t bset_slot(const bset *s, uint64_t v)
+{
+    uint64_t h = v * 0x9E3779B97F4A7C15ull;
+    return (h >> 20) & (s->cap - 1);
+}
+
+/* Adds c; returns 1 when it was not in the set.  Coordinates beyond 21 bits
+ * per axis would collide when packed: they always return 1, so the band
+ * lists every occurrence and the merge's table lookup drops the repeats. */
+static int bset_add(bset *s, i3 c)
+{
+    if (c.x < -(1 << 20) || c.x >= (1 << 20) || c.y < -(1 << 20) || c.y >= (1 << 20) || c.z < -(1 << 20) ||
+        c.z >= (1 << 20))
+        return 1;
+    const uint64_t v = bpack(c) + 1;
+    uint64_t b = bset_slot(s, v);
+    while (s->k[b]) {
+        if (s->k[b] == v)
+            return 0;
+        b = (b + 1) & (s->cap - 1);
+    }
+    s->k[b] = v;
+    if (++s->n * 2 > s->cap) {
+        uint64_t oc = s->cap, *ok = s->k;
+        s->cap *= 2;
+        s->k = (uint64_t *)calloc(s->cap, sizeof(uint64_t));
+        for (uint64_t i = 0; i < oc; i++)
+            if (ok[i]) {
+                uint64_t bb = bset_slot(s, ok[i]);
+                while (s->k[bb])
+                    bb = (bb + 1) & (s->cap - 1);
+                s->k[bb] = ok[i];
+            }
+        free(ok);
+    }
+    return 1;
+}
+
 /* ------------------------------------------------------------------ */
 /* integrate / deintegrate                                              */
 /* ------------------------------------------------------------------ */
@@ -343,28 +417,28 @@ static int update_voxel(const orc_grid *g, float *s, float *w, float *rgb, f3 vm
     if (!(fabsf(sdf) < g->trunc))
         return 0;
     float ww = *w;
+    /* the colour of the pixel the voxel projects to (GPUVoxelGrid.cpp:406-416) */
+    const uint8_t *px = rgb ? rgbin + (g_mut == 10 ? ((int64_t)corj * g->H + cori)
+                                                   : ((int64_t)cori * g->W + corj)) * 3
+                            : NULL;
     if (sign > 0) {
         *s = (*s * ww + sdf) / (ww + 1.0f);
-        if (rgb) {
-            const uint8_t *px = rgbin + ((int64_t)cori * g->W + corj) * 3;
+        if (rgb)
             for (int ch = 0; ch < 3; ch++)
-                rgb[ch] = (rgb[ch] * ww + (float)px[ch]) / (ww + 1.0f);
-        }
+                rgb[ch] = (rgb[ch] * ww + (float)px[ch]) / (ww + (g_mut == 7 ? 2.0f : 1.0f));
         /* Project semantics (no counterpart in the reference, which never
          * caps, GPUVoxelGrid.h:14,62): with max_weight > 0 the weight stops
          * growing at the cap while the running average keeps w+1 as divisor. */
-        if (g->wmax > 0.0f && ww >= g->wmax)
+        if (g->wmax > 0.0f && (g_mut == 8 ? ww > g->wmax : ww >= g->wmax))
             *w = g->wmax;
         else
             *w = ww + 1.0f;
     } else {
         if (ww != 1.0f) {
             *s = (*s * ww - sdf) / (ww - 1.0f);
-            if (rgb) {
-                const uint8_t *px = rgbin + ((int64_t)cori * g->W + corj) * 3;
+            if (rgb)
                 for (int ch = 0; ch < 3; ch++)
-                    rgb[ch] = (rgb[ch] * ww - (float)px[ch]) / (ww - 1.0f);
-            }
+                    rgb[ch] = (rgb[ch] * ww + (g_mut == 11 ? (float)px[ch] : -(float)px[ch])) / (ww - 1.0f);
             *w = ww - 1.0f;
         } else {
             *w = 0.0f;
@@ -390,55 +464,87 @@ int orc_integrate(orc_grid *g, const uint16_t *depth, int pitch, const uint8_t *
     int64_t *touched = (int64_t *)malloc(tcap * sizeof(int64_t));
     const float tr = g->trunc;
 
-    /* vbCollectionTag, GPUVoxelGrid.cpp:471-514 (serial: the allocated set is
-     * order independent; brick indices are not observable). */
-    for (int i = 0; i < g->H; i++) {
-        for (int j = 0; j < g->W; j++) {
-            f3 p = input_pcd(g, depth, pitch, i, j);
-            if (!(p.z > g->rmin))
-                continue;
-            n_valid++;
-            f3 pt = xform(p, E);
-            f3 lo = {pt.x - tr, pt.y - tr, pt.z - tr};
-            f3 hi = {pt.x + tr, pt.y + tr, pt.z + tr};
-            i3 mn = vb_of(lo, g->vbl), mx = vb_of(hi, g->vbl);
-            for (int x = mn.x; x <= mx.x; x++)
-                for (int y = mn.y; y <= mx.y; y++)
-                    for (int z = mn.z; z <= mx.z; z++) {
-                        uint64_t key = orc_prime_hash(x, y, z);
-                        int64_t b = h_find(g, (uint32_t)key);
-                        if (b < 0) {
-                            if (g->n >= g->max_bricks) {
-                                overflow = 1;
-                                continue;
-                            }
-                            if (g->n == g->cap)
-                                bricks_grow(g);
-                            b = g->n++;
+    /* vbCollectionTag, GPUVoxelGrid.cpp:471-514.  The allocated set is order
+     * independent and brick indices are not observable.  Phase 1 (threads
+     * over contiguous row bands) lists each band's distinct blocks in their
+     * order of first occurrence; phase 2 inserts the bands' lists in band
+     * order.  That is the serial pixel order's first-occurrence order, so
+     * even the blocks a full pool refuses are those of a serial pass. */
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    if (nth > g->H)
+        nth = g->H > 0 ? g->H : 1;
+    band_list *bands = (band_list *)calloc((size_t)nth, sizeof(band_list));
+#pragma omp parallel num_threads(nth) reduction(+ : n_valid)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        band_list *bl = &bands[t];
+        const int r0 = (int)((int64_t)g->H * t / nth), r1 = (int)((int64_t)g->H * (t + 1) / nth);
+        bset set;
+        bset_init(&set);
+        for (int i = r0; i < r1; i++) {
+            for (int j = 0; j < g->W; j++) {
+                f3 p = input_pcd(g, depth, pitch, i, j);
+                if (!(p.z > g->rmin))
+                    continue;
+                n_valid++;
+                f3 pt = xform(p, E);
+                f3 lo = {pt.x - tr, pt.y - tr, pt.z - tr};
+                f3 hi = {pt.x + tr, pt.y + tr, pt.z + tr};
+                i3 mn = vb_of(lo, g->vbl), mx = vb_of(hi, g->vbl);
+                for (int x = mn.x; x <= mx.x; x++)
+                    for (int y = mn.y; y <= mx.y; y++)
+                        for (int z = mn.z; z <= mx.z; z++) {
                             i3 c = {x, y, z};
-                            g->coord[b] = c;
-                            g->key[b] = (int64_t)key;
-                            memset(g->sdf + b * VPB, 0, VPB * sizeof(float));
-                            memset(g->w + b * VPB, 0, VPB * sizeof(float));
-                            if (g->with_color)
-                                memset(g->rgb + b * VPB * 3, 0, VPB * 3 * sizeof(float));
-                            g->stamp[b] = 0;
-                            if ((uint64_t)g->n * 2 > g->hcap)
-                                h_grow(g);
-                            h_put(g, (uint32_t)key, b);
-                            n_new++;
+                            if (bset_add(&set, c))
+                                band_push(bl, c);
                         }
-                        if (g->stamp[b] != g->epoch) {
-                            g->stamp[b] = g->epoch;
-                            if (tn == tcap) {
-                                tcap *= 2;
-                                touched = (int64_t *)realloc(touched, tcap * sizeof(int64_t));
-                            }
-                            touched[tn++] = b;
-                        }
-                    }
+            }
         }
+        bset_free(&set);
     }
+    for (int t = 0; t < nth; t++) {
+        for (int64_t e = 0; e < bands[t].n; e++) {
+            const i3 c = bands[t].v[e];
+            uint64_t key = orc_prime_hash(c.x, c.y, c.z);
+            int64_t b = h_find(g, (uint32_t)key);
+            if (b < 0) {
+                if (g->n >= g->max_bricks) {
+                    overflow = 1;
+                    continue;
+                }
+                if (g->n == g->cap)
+                    bricks_grow(g);
+                b = g->n++;
+                g->coord[b] = c;
+                g->key[b] = (int64_t)key;
+                memset(g->sdf + b * VPB, 0, VPB * sizeof(float));
+                memset(g->w + b * VPB, 0, VPB * sizeof(float));
+                if (g->with_color)
+                    memset(g->rgb + b * VPB * 3, 0, VPB * 3 * sizeof(float));
+                g->stamp[b] = 0;
+                if ((uint64_t)g->n * 2 > g->hcap)
+                    h_grow(g);
+                h_put(g, (uint32_t)key, b);
+                n_new++;
+            }
+            if (g->stamp[b] != g->epoch) {
+                g->stamp[b] = g->epoch;
+                if (tn == tcap) {
+                    tcap *= 2;
+                    touched = (int64_t *)realloc(touched, tcap * sizeof(int64_t));
+                }
+                touched[tn++] = b;
+            }
+        }
+        free(bands[t].v);
+    }
+    free(bands);
 
     /* integrationCore[Color]Tag / deIntegrationCore[Color]Tag,
      * GPUVoxelGrid.cpp:516-550: vm = pzero + (k + 0.5) * vl in double. */
@@ -632,9 +738,11 @@ static int march(const orc_grid *g, int i, int j, const float *Er, f3 *pout, f3 
                     pout->z = rv.z + s * dir.z;
                     if (g->with_color) {
                         const float *co = g->rgb + ovi * 3, *cc = g->rgb + cvi * 3;
-                        cout->x = (float)((0.5 * (double)co[0] + 0.5 * (double)cc[0]) / 255.0);
-                        cout->y = (float)((0.5 * (double)co[1] + 0.5 * (double)cc[1]) / 255.0);
-                        cout->z = (float)((0.5 * (double)co[2] + 0.5 * (double)cc[2]) / 255.0);
+                        /* mutation 9 weights the inner voxel only */
+                        const double wo = g_mut == 9 ? 0.0 : 0.5, wi = g_mut == 9 ? 1.0 : 0.5;
+                        cout->x = (float)((wo * (double)co[0] + wi * (double)cc[0]) / 255.0);
+                        cout->y = (float)((wo * (double)co[1] + wi * (double)cc[1]) / 255.0);
+                        cout->z = (float)((wo * (double)co[2] + wi * (double)cc[2]) / 255.0);
                     }
                 } else {
                     Fn = 1;
@@ -754,6 +862,14 @@ int64_t orc_clean(orc_grid *g)
     for (int64_t b = 0; b < g->n; b++)
         h_put(g, (uint32_t)g->key[b], b);
     return removed;
+}
+
+/* Test hook: overwrites the colours of every brick (export order,
+ * [n][512][3]), so a test can give voxels known colours before a raycast. */
+void orc_import_rgb(orc_grid *g, const float *rgb)
+{
+    if (g->with_color && rgb)
+        memcpy(g->rgb, rgb, (size_t)g->n * VPB * 3 * sizeof(float));
 }
 
 void orc_export(const orc_grid *g, int32_t *coords, int64_t *keys, float *sdf, float *w, float *rgb)

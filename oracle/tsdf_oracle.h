@@ -84,6 +84,7 @@ int64_t  orc_n_bricks(const orc_grid *g);
 /* cleanMemory (GPUVoxelGrid.cpp:185-196): erases the bricks whose weights are
  * all zero; returns how many were erased. */
 int64_t  orc_clean(orc_grid *g);
+void     orc_import_rgb(orc_grid *g, const float *rgb);
 /* coords: n*3 int32, keys: n int64 (sign-extended reference hash),
  * sdf/w: n*512 floats, rgb: n*512*3 floats or NULL.  Brick order = insertion
  * order; voxel order x + 8y + 64z (GPUVoxelGrid.h:28-31). */

@@ -28,7 +28,7 @@ RAY_UB_MISSING_OUT = 2
 
 # Every symbol include/tsdf.h declares (checked by tests/test_capi_exports.py).
 EXPORTS = (
-    "tsdf_default_params", "tsdf_create", "tsdf_destroy", "tsdf_reset",
+    "tsdf_default_params", "tsdf_create", "tsdf_get_params", "tsdf_destroy", "tsdf_reset",
     "tsdf_set_intrinsics", "tsdf_integrate", "tsdf_integrate_batch",
     "tsdf_reintegrate_batch", "tsdf_reserve_batch", "tsdf_raycast", "tsdf_sync", "tsdf_stats",
     "tsdf_export_bricks", "tsdf_pose_prepare", "tsdf_status_string",
@@ -37,7 +37,7 @@ EXPORTS = (
     "tsdf_render", "tsdf_render_backward", "tsdf_track_normal_eq", "tsdf_se3_gauss_newton_step",
     "tsdf_clean", "tsdf_extract_mesh",
     "tsdf_icp_default_params", "tsdf_depth_to_points", "tsdf_points_to_normals", "tsdf_icp_associate",
-    "tsdf_icp_moments", "tsdf_icp",
+    "tsdf_icp_moments", "tsdf_icp", "tsdf_icp_release",
     "tsdf_mat_to_dof", "tsdf_dof_to_mat", "tsdf_reint_diff", "tsdf_reint_select",
 )
 
@@ -139,6 +139,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_default_params.argtypes = [P(TsdfParams)]
     L.tsdf_default_params.restype = None
     L.tsdf_create.argtypes = [P(TsdfParams), P(vp)]
+    L.tsdf_get_params.argtypes = [vp, P(TsdfParams)]
     L.tsdf_destroy.argtypes = [vp]
     L.tsdf_destroy.restype = None
     L.tsdf_reset.argtypes = [vp, vp]
@@ -161,6 +162,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_icp_associate.argtypes = [vp, vp, vp, fp, P(TsdfView), P(TsdfIcpParams), vp, vp]
     L.tsdf_icp_moments.argtypes = [vp, vp, dp, C.c_int32, C.c_int32, dp, vp]
     L.tsdf_icp.argtypes = [vp, vp, dp, dp, P(TsdfView), P(TsdfIcpParams), dp, fp, fp, vp]
+    L.tsdf_icp_release.argtypes = []
     L.tsdf_mat_to_dof.argtypes = [dp, dp]
     L.tsdf_dof_to_mat.argtypes = [dp, dp]
     L.tsdf_reint_diff.argtypes = [dp, dp]
@@ -182,7 +184,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_status_string.restype = C.c_char_p
     L.tsdf_last_error.argtypes = []
     L.tsdf_last_error.restype = C.c_char_p
-    for name in ("tsdf_create", "tsdf_reset", "tsdf_set_intrinsics", "tsdf_integrate",
+    for name in ("tsdf_create", "tsdf_get_params", "tsdf_reset", "tsdf_set_intrinsics", "tsdf_integrate",
                  "tsdf_integrate_batch", "tsdf_reintegrate_batch", "tsdf_reserve_batch", "tsdf_raycast",
                  "tsdf_sync", "tsdf_stats", "tsdf_export_bricks", "tsdf_pose_prepare",
                  "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",

@@ -94,6 +94,7 @@ struct BatchStage {
     OpDesc ops[kMaxOps];
 };
 constexpr int kMaxAxisBlocks = 64;
+constexpr float kF16MaxWeight = 2048.0f;  // largest weight fp16 counts exactly
 
 // ---------------------------------------------------------------------------
 // small device helpers
@@ -1645,7 +1646,7 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
         return fail(TSDF_BAD_ARG, "max_ray_length out of range");
     if (!(p->min_depth >= 0.0f) || !(p->max_weight >= 0.0f))
         return fail(TSDF_BAD_ARG, "min_depth and max_weight must be >= 0");
-    if (p->storage == TSDF_STORAGE_F16 && p->max_weight > 2048.0f)
+    if (p->storage == TSDF_STORAGE_F16 && p->max_weight > kF16MaxWeight)
         return fail(TSDF_BAD_ARG, "fp16 weights are exact only up to 2048");
 
     DeviceScope ds(p->device);
@@ -1654,6 +1655,10 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
     tsdf_grid* g = new tsdf_grid();
     memset(g, 0, sizeof(*g));
     g->p = *p;
+    // fp16 holds every integer weight up to 2048 and no larger one (2049
+    // rounds to 2048), so an uncapped fp16 grid is capped there
+    if (p->storage == TSDF_STORAGE_F16 && p->max_weight == 0.0f)
+        g->p.max_weight = kF16MaxWeight;
     g->vbl = (float)kVPE * vl;  // GPUVoxelGrid.cpp:15
     for (int k = 0; k < kVPE; ++k)
         g->off.v[k] = ((double)k + 0.5) * (double)vl;
@@ -1798,6 +1803,14 @@ tsdf_status tsdf_reset(tsdf_grid* g, void* stream)
     return TSDF_OK;
 }
 
+tsdf_status tsdf_get_params(const tsdf_grid* g, tsdf_params* p)
+{
+    if (!g || !p)
+        return fail(TSDF_BAD_ARG, "NULL grid or params");
+    *p = g->p;
+    return TSDF_OK;
+}
+
 tsdf_status tsdf_set_intrinsics(tsdf_grid* g, float fx, float fy, float cx, float cy)
 {
     if (!g)
@@ -1825,13 +1838,15 @@ tsdf_status tsdf_integrate(tsdf_grid* g, const uint16_t* depth, int32_t pitch, c
         return cs;
     if (sign != 1 && sign != -1)
         return fail(TSDF_BAD_ARG, "sign must be +1 or -1");
+    if (rgb && !g->p.with_color)
+        return fail(TSDF_BAD_ARG, "colour given to a grid created without colour storage");
     if (!pose_ok(E))
         return fail(TSDF_BAD_POSE, "pose must be finite and rigid with last row 0 0 0 1");
     DeviceScope ds(g->p.device);
     hipStream_t s = (hipStream_t)stream;
     OpDesc op;
     op.depth = depth;
-    op.rgb = (g->p.with_color && rgb) ? rgb : nullptr;
+    op.rgb = rgb;
     op.ptmap = g->ptmap;
     op.pitch = pitch;
     op.sign = sign;
@@ -1928,6 +1943,8 @@ tsdf_status tsdf_integrate_batch(tsdf_grid* g, int32_t n_ops, const uint16_t* co
         return fail(TSDF_NO_INTRINSICS, "tsdf_set_intrinsics was not called");
     if (pitch < g->p.width)
         return fail(TSDF_BAD_DIMS, "pitch smaller than the grid width");
+    if (rgbs && !g->p.with_color)
+        return fail(TSDF_BAD_ARG, "colour given to a grid created without colour storage");
     for (int k = 0; k < n_ops; ++k) {
         if (!depths[k])
             return fail(TSDF_BAD_ARG, "depth pointer of an op is NULL");

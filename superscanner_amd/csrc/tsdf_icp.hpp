@@ -611,6 +611,27 @@ tsdf_status tsdf_icp_moments(const float* x_points, const float* x_nearest, cons
     return TSDF_OK;
 }
 
+tsdf_status tsdf_icp_release(void)
+{
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kIcpMaxDevices)
+        return fail(TSDF_BAD_ARG, "device index beyond the ICP workspace table");
+    IcpWorkspace& ws = g_icp_ws[dev];
+    std::lock_guard<std::mutex> lock(ws.mu);
+    // tsdf_icp returns only after its stream drained, so no kernel of a
+    // finished call still uses the buffers
+    (void)hipFree(ws.buf);
+    (void)hipFree(ws.mom);
+    if (ws.mom_host)
+        (void)hipHostFree(ws.mom_host);
+    ws.buf = nullptr;
+    ws.mom = nullptr;
+    ws.mom_host = nullptr;
+    ws.n_cap = 0;
+    return TSDF_OK;
+}
+
 tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const double tx[16], const double ty[16],
                      const tsdf_view* cam, const tsdf_icp_params* p, double T_out[16], float* rmse_out,
                      float* fitness_out, void* stream)
@@ -629,6 +650,14 @@ tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const doubl
         return fail(TSDF_BAD_ARG, "device index beyond the ICP workspace table");
     IcpWorkspace& ws = g_icp_ws[dev];
     std::lock_guard<std::mutex> lock(ws.mu);
+    // Every exit after the first launch waits for the stream before the lock
+    // is released: the kernels read the caller's depth images and write the
+    // shared workspace, and the call is synchronous (declared after `lock`,
+    // so it runs first on the way out).
+    struct StreamDrain {
+        hipStream_t s;
+        ~StreamDrain() { (void)hipStreamSynchronize(s); }
+    } drain{s};
     if (!icp_workspace(ws, n))
         return fail(TSDF_HIP_ERROR, "icp workspace allocation failed");
     float* buf = ws.buf;
@@ -691,6 +720,10 @@ tsdf_status tsdf_icp(const uint16_t* depth1, const uint16_t* depth2, const doubl
     }
     if (r != TSDF_OK)
         return r;
+    // the launches of an n_iterations == 0 call end here too
+    const hipError_t se = hipStreamSynchronize(s);
+    if (se != hipSuccess)
+        return fail(TSDF_HIP_ERROR, std::string("icp: ") + hipGetErrorString(se));
     for (int k = 0; k < 16; ++k)
         T_out[k] = best[k];
     if (rmse_out)

@@ -140,3 +140,9 @@ def kokkosICP(d1, d2, tx, ty, intr, ICPType=PointToPoint, n_it=10, n_search=5, m
                                TY.ctypes.data_as(dp), C.byref(v), C.byref(p), T.ctypes.data_as(dp),
                                C.byref(rmse), C.byref(fit), _stream(stream)), "tsdf_icp")
     return T.reshape(4, 4), float(rmse.value), float(fit.value)
+
+
+def release() -> None:
+    """Frees the workspace kokkosICP keeps on the current device between
+    calls (tsdf_icp_release); the next call allocates it again."""
+    check(_lib.load().tsdf_icp_release(), "tsdf_icp_release")

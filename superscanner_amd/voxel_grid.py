@@ -111,6 +111,12 @@ class GPUVoxelGrid:
         except Exception:
             pass
 
+    def get_params(self) -> dict:
+        """The parameters the grid runs with (tsdf_get_params)."""
+        p = TsdfParams()
+        check(self._L.tsdf_get_params(self._h, C.byref(p)), "tsdf_get_params")
+        return {n: getattr(p, n) for n, _ in TsdfParams._fields_}
+
     def reset(self, stream=None):
         check(self._L.tsdf_reset(self._h, _stream_ptr(stream)), "tsdf_reset")
 

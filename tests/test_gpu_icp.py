@@ -119,6 +119,34 @@ def test_kokkos_icp_keeps_an_exact_alignment(gpu):
     assert np.abs(T - Rz).max() < 1e-6 and rmse < 1e-6 and fit > 0.5
 
 
+def test_kokkos_icp_workspace_and_streams(gpu):
+    """The per-device workspace: released, then grown from a small frame to
+    a larger one (both follow the oracle); a call with no iteration returns
+    the identity and waits for its own launches; a call on a second stream
+    after it, and a call after another release, give the same bits."""
+    icp.release()
+    runs = []
+    for h, w in ((97, 131), (240, 320)):
+        d1, d2, T1, T2, intr = frames(h, w, noise=1.0)
+        start = T1 @ np.array([[1, 0, 0, 0.004], [0, 1, 0, -0.003], [0, 0, 1, 0.002], [0, 0, 0, 1.0]])
+        T, rmse, fit = icp.kokkosICP(d1, d2, start, T2, intr)
+        To, rmse_o, fit_o, rec = oicp.kokkosICP(d1, d2, start, T2, intr)
+        assert abs(rmse - rmse_o) < 1e-4 * rmse_o and abs(fit - fit_o) < 1e-6, (h, w)
+        near = [r["T"] for r in rec if r["rmse"] <= rmse_o * (1 + 1e-4)]
+        assert min(np.abs(T - t).max() for t in near) < 1e-5, (h, w)
+        runs.append((d1, d2, start, T2, intr, T, rmse, fit))
+    d1, d2, start, T2, intr, T, rmse, fit = runs[-1]
+    T0, r0, f0 = icp.kokkosICP(d1, d2, start, T2, intr, n_it=0)
+    assert np.array_equal(T0, np.eye(4)) and r0 == 1e6 and f0 == 0.0
+    s2 = torch.cuda.Stream()
+    with torch.cuda.stream(s2):
+        Tb, rb, fb = icp.kokkosICP(d1, d2, start, T2, intr, stream=s2)
+    assert np.array_equal(Tb, T) and rb == rmse and fb == fit
+    icp.release()
+    Tc, rc, fc = icp.kokkosICP(d1, d2, start, T2, intr)
+    assert np.array_equal(Tc, T) and rc == rmse and fc == fit
+
+
 def test_argument_errors(gpu):
     from superscanner_amd import _lib
     d = torch.zeros((8, 8), dtype=torch.int16, device=gpu)
