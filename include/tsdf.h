@@ -349,6 +349,18 @@ tsdf_status tsdf_icp_associate(const float *x_points, const float *y_points, con
 tsdf_status tsdf_icp_moments(const float *x_points, const float *x_nearest, const double Txy[16],
                              int32_t height, int32_t width, double *moments, void *stream);
 
+/* The sums of tsdf_icp_moments, bit-identical to them, for callers that
+ * cannot allocate device memory (tight tracking loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It allocates nothing: its
+ * partial sums live in a fixed 73 872-byte static device array (one per
+ * device, held until the process ends), so it takes any image of at most
+ * 2^29 - 1 pixels and refuses larger ones with TSDF_BAD_ARG.  Calls on one
+ * device share that array: overlapping calls would race on it, so the
+ * library serialises them with a per-device lock held until the call's
+ * stream has drained.  Synchronous. */
+tsdf_status tsdf_icp_moments_noalloc(const float *x_points, const float *x_nearest, const double Txy[16],
+                                     int32_t height, int32_t width, double *moments, void *stream);
+
 /* kokkosICP(space, d1, d2, tx, ty, type, &rsme), kokkosICP.cpp:23-191:
  * aligns the points of depth1 (pose tx, camera->world) to those of depth2
  * (pose ty), starting from Txy = ty^-1 tx; each iteration associates, takes

@@ -37,7 +37,7 @@ EXPORTS = (
     "tsdf_render", "tsdf_render_backward", "tsdf_track_normal_eq", "tsdf_se3_gauss_newton_step",
     "tsdf_clean", "tsdf_extract_mesh",
     "tsdf_icp_default_params", "tsdf_depth_to_points", "tsdf_points_to_normals", "tsdf_icp_associate",
-    "tsdf_icp_moments", "tsdf_icp", "tsdf_icp_release",
+    "tsdf_icp_moments", "tsdf_icp_moments_noalloc", "tsdf_icp", "tsdf_icp_release",
     "tsdf_mat_to_dof", "tsdf_dof_to_mat", "tsdf_reint_diff", "tsdf_reint_select",
 )
 
@@ -161,6 +161,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_points_to_normals.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
     L.tsdf_icp_associate.argtypes = [vp, vp, vp, fp, P(TsdfView), P(TsdfIcpParams), vp, vp]
     L.tsdf_icp_moments.argtypes = [vp, vp, dp, C.c_int32, C.c_int32, dp, vp]
+    L.tsdf_icp_moments_noalloc.argtypes = [vp, vp, dp, C.c_int32, C.c_int32, dp, vp]
     L.tsdf_icp.argtypes = [vp, vp, dp, dp, P(TsdfView), P(TsdfIcpParams), dp, fp, fp, vp]
     L.tsdf_icp_release.argtypes = []
     L.tsdf_mat_to_dof.argtypes = [dp, dp]

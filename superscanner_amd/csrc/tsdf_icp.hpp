@@ -17,10 +17,20 @@
 
 #include <mutex>
 
+// Scratch of tsdf_icp_moments_noalloc: the per-workgroup partial sums and the
+// final sums of k_icp_moments, (512 + 1) x 18 doubles = 73 872 bytes.  A
+// static device array has one copy per device, each at its own address, held
+// until the process ends.  On the host its name is only a stand-in: kernels
+// and copies get the address hipGetSymbolAddress returns for the current
+// device (icp_scratch_ptr).
+static __device__ __align__(256) double scratch[(512 + 1) * TSDF_ICP_MOMENTS];
+
 namespace {
 
 constexpr int kIcpMoments = TSDF_ICP_MOMENTS;
 constexpr int kIcpBlocks = 512;  // partial sums of k_icp_moments
+static_assert(sizeof(scratch) == (size_t)(kIcpBlocks + 1) * kIcpMoments * sizeof(double),
+              "the no-allocation scratch holds the partial and final sums");
 
 __device__ __forceinline__ f3 ld3(const float* p, unsigned int k)
 {
@@ -426,6 +436,15 @@ bool icp_workspace(IcpWorkspace& w, size_t n)
     return true;
 }
 
+// Device address of `scratch` on each device, looked up once per device (with
+// that device current) by the first tsdf_icp_moments_noalloc call there.  The
+// mutex also serialises the calls on one device: they share its one array.
+struct IcpScratch {
+    std::mutex mu;
+    double* ptr = nullptr;
+};
+IcpScratch g_icp_scratch[kIcpMaxDevices];
+
 // 4x4 inverse by cofactors (double, row-major); false when singular.
 bool inv4(const double* m, double* out)
 {
@@ -608,6 +627,43 @@ tsdf_status tsdf_icp_moments(const float* x_points, const float* x_nearest, cons
     (void)hipFree(buf);
     if (e != hipSuccess)
         return fail(TSDF_HIP_ERROR, std::string("icp moments: ") + hipGetErrorString(e));
+    return TSDF_OK;
+}
+
+tsdf_status tsdf_icp_moments_noalloc(const float* x_points, const float* x_nearest, const double Txy[16],
+                                     int32_t height, int32_t width, double* moments, void* stream)
+{
+    if (!x_points || !x_nearest || !Txy || !moments || height <= 0 || width <= 0)
+        return fail(TSDF_BAD_ARG, "NULL buffer or non-positive size");
+    if ((int64_t)height * width >= (1ll << 29))
+        return fail(TSDF_BAD_ARG, "image larger than the no-allocation sums take (at most 2^29 - 1 pixels)");
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kIcpMaxDevices)
+        return fail(TSDF_BAD_ARG, "device index beyond the scratch table");
+    IcpScratch& sc = g_icp_scratch[dev];
+    std::lock_guard<std::mutex> lock(sc.mu);
+    if (!sc.ptr) {
+        void* p = nullptr;
+        const hipError_t ge = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (ge != hipSuccess || !p)
+            return fail(TSDF_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(ge));
+        sc.ptr = (double*)p;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // every partial and the final sums are written before they are read, so
+    // the previous call's contents need no clearing
+    launch_icp_moments(x_points, x_nearest, Txy, (unsigned int)height * (unsigned int)width, sc.ptr, s);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(moments, sc.ptr + (size_t)kIcpBlocks * kIcpMoments, kIcpMoments * sizeof(double),
+                           hipMemcpyDeviceToHost, s);
+    // the lock is held until the stream drained: the next call may reuse the array
+    const hipError_t se = hipStreamSynchronize(s);
+    if (e == hipSuccess)
+        e = se;
+    if (e != hipSuccess)
+        return fail(TSDF_HIP_ERROR, std::string("icp moments (no allocation): ") + hipGetErrorString(e));
     return TSDF_OK;
 }
 

@@ -121,6 +121,21 @@ def moments(x, x_nearest, Txy, stream=None) -> np.ndarray:
     return out
 
 
+def moments_noalloc(x, x_nearest, Txy, stream=None) -> np.ndarray:
+    """The sums of ``moments`` (bit-identical) through the entry point that
+    allocates no device memory (tsdf_icp_moments_noalloc)."""
+    x = _dev_pts(x)
+    h, w = x.shape[:2]
+    xn = _dev_pts(x_nearest, h, w)
+    T = np.ascontiguousarray(np.asarray(Txy, np.float64).reshape(16))
+    out = np.zeros(_lib.ICP_MOMENTS, np.float64)
+    dp = C.POINTER(C.c_double)
+    check(_lib.load().tsdf_icp_moments_noalloc(C.c_void_p(x.data_ptr()), C.c_void_p(xn.data_ptr()),
+                                               T.ctypes.data_as(dp), h, w, out.ctypes.data_as(dp),
+                                               _stream(stream)), "tsdf_icp_moments_noalloc")
+    return out
+
+
 def kokkosICP(d1, d2, tx, ty, intr, ICPType=PointToPoint, n_it=10, n_search=5, max_dist=0.075, stream=None):
     """kokkosICP(space, d1, d2, tx, ty, ICPType, &rsme), kokkosICP.cpp:23-191:
     the transform taking the points of d1 (pose tx) onto those of d2 (pose
