@@ -362,9 +362,9 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
                 ++nvalid;
                 const f3 pw = xform(p, op.pose.E);
                 o = make_float4(pw.x, pw.y, pw.z, 1.0f);
-                const int bx0 = div_index(pw.x - g.trunc, g.vbl, g.rvbl), bx1 = div_index(pw.x + g.trunc, g.vbl, g.rvbl);
-                const int by0 = div_index(pw.y - g.trunc, g.vbl, g.rvbl), by1 = div_index(pw.y + g.trunc, g.vbl, g.rvbl);
-                const int bz0 = div_index(pw.z - g.trunc, g.vbl, g.rvbl), bz1 = div_index(pw.z + g.trunc, g.vbl, g.rvbl);
+                const int bx0 = vb_index_q(pw.x - g.trunc, g.Rvbl), bx1 = vb_index_q(pw.x + g.trunc, g.Rvbl);
+                const int by0 = vb_index_q(pw.y - g.trunc, g.Rvbl), by1 = vb_index_q(pw.y + g.trunc, g.Rvbl);
+                const int bz0 = vb_index_q(pw.z - g.trunc, g.Rvbl), bz1 = vb_index_q(pw.z + g.trunc, g.Rvbl);
                 const bool sane = (long long)bx1 - bx0 < kMaxAxisBlocks && (long long)by1 - by0 < kMaxAxisBlocks &&
                                   (long long)bz1 - bz0 < kMaxAxisBlocks;
                 const bool pk = packable(bx0) && packable(bx1) && packable(by0) && packable(by1) && packable(bz0) &&
@@ -1334,6 +1334,7 @@ GridView view(const tsdf_grid* g)
     v.vl = g->p.voxel_length;
     v.vbl = g->vbl;
     v.rvbl = 1.0f / g->vbl;
+    v.Rvbl = 1.0 / (double)g->vbl;
     v.trunc = g->p.truncation;
     v.rmin = g->p.min_depth;
     v.dmax = g->p.max_ray_length;
@@ -1508,8 +1509,8 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
     rp.vbl = g->vbl;
     rp.dt = g->p.truncation;
     rp.dmax = g->p.max_ray_length;
-    rp.rvl = 1.0f / rp.vl;
-    rp.rvbl = 1.0f / rp.vbl;
+    rp.Rvl = 1.0 / (double)rp.vl;
+    rp.Rvbl = 1.0 / (double)rp.vbl;
     rp.back = rp.vbl - rp.dt;
     rp.H = g->p.height;
     rp.W = g->p.width;

@@ -76,7 +76,8 @@ struct GridView {
     uint32_t mask;              // n_buckets - 1
     uint32_t n_bricks;
     float vl, vbl, trunc, rmin, dmax, wmax;
-    float rvbl;  // 1 / vbl rounded (div_index)
+    float rvbl;    // 1 / vbl rounded (render)
+    double Rvbl;   // RN64(1 / vbl): exact block index through quot_rn
     int H, W;
     Camera cam;
 };
@@ -246,7 +247,7 @@ inline void fp_voxel(const Footprint&, int, int) {}
 // Raycast constants of one call.
 struct RayParams {
     float vl, vbl, dt, dmax;
-    float rvl, rvbl;  // 1 / vl, 1 / vbl rounded (div_fast)
+    double Rvl, Rvbl;  // RN64(1 / vl), RN64(1 / vbl): exact indices through quot_rn
     float back;       // vbl - d_trunc, GPUVoxelGrid.cpp:613
     int H, W;
     int xcd_map;      // 1: each XCD takes one band of the image
@@ -258,34 +259,33 @@ struct RayParams {
     int df_lo[3], df_dim[3];
 };
 
-// getVBIndeces of a point: the fast path for all three axes, one shared
-// fallback to the reference arithmetic.
-__host__ __device__ __forceinline__ void vb3(f3 r, const RayParams& rp, int& x, int& y, int& z)
+// The reference's index rule on a quotient q = RN32(a / d) (exact through
+// quot_rn): (int)q for a >= 0, (int)(q - 1) below.  A position on a ray lies
+// within the pose translation plus d_max of the origin (|t| <= 1e5 m and
+// d_max <= 1e3 m are enforced), so |q| stays far below 2^31 and the plain
+// conversion truncates exactly like the reference's (int).
+__host__ __device__ __forceinline__ int ray_index(float a, double R)
 {
-    bool a, b, c;
-    x = div_fast(r.x, rp.rvbl, a);
-    y = div_fast(r.y, rp.rvbl, b);
-    z = div_fast(r.z, rp.rvbl, c);
-    if (!(a && b && c)) {
-        x = vb_index(r.x, rp.vbl);
-        y = vb_index(r.y, rp.vbl);
-        z = vb_index(r.z, rp.vbl);
-    }
+    const float q = quot_rn(a, R);
+    return (int)((a >= 0) ? q : q - 1.0f);
 }
 
-// getVinVBIndeces of a point in block (bx, by, bz), same scheme.
+// getVBIndeces of a point, GPUVoxelGrid.h:162-190.
+__host__ __device__ __forceinline__ void vb3(f3 r, const RayParams& rp, int& x, int& y, int& z)
+{
+    x = ray_index(r.x, rp.Rvbl);
+    y = ray_index(r.y, rp.Rvbl);
+    z = ray_index(r.z, rp.Rvbl);
+}
+
+// getVinVBIndeces of a point in block (bx, by, bz), GPUVoxelGrid.h:192-221
+// (p - vbl * vbInd in float, kokkosFunctions.h:737-740).
 __host__ __device__ __forceinline__ void vin3(f3 r, int bx, int by, int bz, const RayParams& rp, int& x, int& y,
                                               int& z)
 {
-    bool a, b, c;
-    x = div_fast(r.x - rp.vbl * (float)bx, rp.rvl, a);
-    y = div_fast(r.y - rp.vbl * (float)by, rp.rvl, b);
-    z = div_fast(r.z - rp.vbl * (float)bz, rp.rvl, c);
-    if (!(a && b && c)) {
-        x = vin_index(r.x, bx, rp.vbl, rp.vl);
-        y = vin_index(r.y, by, rp.vbl, rp.vl);
-        z = vin_index(r.z, bz, rp.vbl, rp.vl);
-    }
+    x = ray_index(r.x - rp.vbl * (float)bx, rp.Rvl);
+    y = ray_index(r.y - rp.vbl * (float)by, rp.Rvl);
+    z = ray_index(r.z - rp.vbl * (float)bz, rp.Rvl);
 }
 
 struct RayResult {

@@ -95,6 +95,40 @@ __host__ __device__ __forceinline__ int div_index(float a, float d, float rd)
     return (a >= 0) ? f2i(a / d) : f2i(a / d - 1.0f);
 }
 
+// RN32(a / d) exactly, from R = RN64(1 / (double)d) (d a positive normal
+// float), for every float a:
+//  - t = RN64((double)a * R) is within |a / d| 2^-52 (1 + 2^-54) of a / d;
+//  - a / d is never the midpoint mu of two consecutive floats (a = mu d would
+//    give a the odd part of mu, at least 2^24 + 1), and otherwise
+//    a - mu d is a nonzero multiple of 2^min(e_a, e_mu + e_d) (exponents of
+//    the integer significands), which puts a / d at least 2^-50 |a / d| away
+//    from every midpoint;
+// so no midpoint lies between t and a / d, and RN32(t) = RN32(a / d).
+// Quotients below 2^-126 may round differently; every index computed from
+// them (vb_index_q, vin_index_q) truncates them to the same integer.
+// Infinities and NaN propagate as in the division.
+__host__ __device__ __forceinline__ float quot_rn(float a, double R)
+{
+    return (float)((double)a * R);
+}
+
+// getVBIndeces per axis (GPUVoxelGrid.h:162-190) from the exact quotient:
+// equal to vb_index(a, d) for every a (R = RN64(1 / d)).
+__host__ __device__ __forceinline__ int vb_index_q(float a, double R)
+{
+    const float q = quot_rn(a, R);
+    return (a >= 0) ? f2i(q) : f2i(q - 1.0f);
+}
+
+// getVinVBIndeces per axis (GPUVoxelGrid.h:192-221) from the exact quotient:
+// equal to vin_index(p, vb, vbl, vl) for every p (Rvl = RN64(1 / vl)).
+__host__ __device__ __forceinline__ int vin_index_q(float p, int vb, float vbl, double Rvl)
+{
+    const float q = p - vbl * (float)vb;
+    const float t = quot_rn(q, Rvl);
+    return (q >= 0) ? f2i(t) : f2i(t - 1.0f);
+}
+
 // Certifies an approximate reciprocal r of z: e = RN(r z - 1) (one fma)
 // with |e| <= 2^-22 bounds the relative error of r by 2^-22 (1 + 2^-23),
 // whatever produced r (0, Inf, NaN and flushed denormals all fail).

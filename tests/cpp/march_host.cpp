@@ -80,8 +80,8 @@ extern "C" int march_host(int H, int W, float fx, float fy, float cx, float cy, 
     rp.vbl = g.vbl;
     rp.dt = trunc;
     rp.dmax = dmax;
-    rp.rvl = 1.0f / vl;
-    rp.rvbl = 1.0f / rp.vbl;
+    rp.Rvl = 1.0 / (double)vl;
+    rp.Rvbl = 1.0 / (double)rp.vbl;
     rp.back = rp.vbl - rp.dt;
     rp.H = H;
     rp.W = W;

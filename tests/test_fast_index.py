@@ -1,5 +1,7 @@
 """The exact fast paths of the block / voxel index (tsdf_math.hpp div_index,
-tests/cpp/test_div_index.cpp) and of the integrate kernel's pixel index
+vb_index_q / vin_index_q and the quotient quot_rn, checked on random and
+adversarial inputs and on every float in [2^-20, 2^8) m of both signs for six
+divisors; tests/cpp/test_div_index.cpp) and of the integrate kernel's pixel index
 (pix_axis_fast, tests/cpp/test_pix_index.cpp) agree with the reference
 arithmetic on millions of random and adversarial inputs (host builds)."""
 import os
@@ -19,9 +21,10 @@ def test_div_index_matches_reference(tmp_path):
     subprocess.run([hipcc, "-O2", "-std=c++17", "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
                     "-o", str(exe), os.path.join(ROOT, "tests", "cpp", "test_div_index.cpp")],
                    check=True, capture_output=True)
-    r = subprocess.run([str(exe), "20000000"], capture_output=True, text=True, timeout=600)
+    r = subprocess.run([str(exe), "20000000", "sweep"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "agree with the reference arithmetic" in r.stdout
+    assert "quotients equal to the IEEE division" in r.stdout
 
 
 @pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
