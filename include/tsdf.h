@@ -109,7 +109,10 @@ typedef struct {
  * per device and workload by superscanner_amd.autotune). */
 typedef struct {
     int32_t raycast_xcd_map;      /* 1: each XCD takes one band of the image    */
-    int32_t raycast_superblocks;  /* 1: skip empty superblocks when certified   */
+    int32_t raycast_skip;         /* empty-space skip of the raycast (results   */
+                                  /* do not depend on it): 0 none, 1 distance   */
+                                  /* field over superblocks, 2 depth bounds of  */
+                                  /* 8 x 8-pixel tiles from the projected bricks */
     int32_t integrate_wg_per_cu;  /* brick-update workgroups per CU            */
     int32_t batch_ops_per_pass;   /* ops per collection + update pass of a     */
                                   /* batch, 1..256                              */
@@ -119,6 +122,10 @@ typedef struct {
                                   /* in (results do not depend on it): 0 the    */
                                   /* hardware's, 1 reversed, 2 the tiles that   */
                                   /* took longest in the previous raycast first */
+    int32_t raycast_budget;       /* march rounds of a ray before it is handed  */
+                                  /* to a second kernel that carries each ray   */
+                                  /* with 16 lanes (results do not depend on    */
+                                  /* it); 0 = never, at most 4096               */
 } tsdf_tuning;
 
 /* Camera of a render view (the differentiable render has its own resolution

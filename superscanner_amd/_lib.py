@@ -85,9 +85,9 @@ class TsdfStats(C.Structure):
 
 
 class TsdfTuning(C.Structure):
-    _fields_ = [("raycast_xcd_map", C.c_int32), ("raycast_superblocks", C.c_int32),
+    _fields_ = [("raycast_xcd_map", C.c_int32), ("raycast_skip", C.c_int32),
                 ("integrate_wg_per_cu", C.c_int32), ("batch_ops_per_pass", C.c_int32),
-                ("batch_graph", C.c_int32), ("raycast_order", C.c_int32)]
+                ("batch_graph", C.c_int32), ("raycast_order", C.c_int32), ("raycast_budget", C.c_int32)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

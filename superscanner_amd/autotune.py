@@ -11,7 +11,10 @@ The tunables are the fields of ``tsdf_tuning`` (include/tsdf.h):
                         payload round trips)
   batch_graph           a batch of <= 256 ops as one captured graph
   raycast_xcd_map       XCD-banded tile order of the raycast
-  raycast_superblocks   certified empty-superblock skipping in the raycast
+  raycast_skip          exact empty-space skip of the raycast (0 none, 1 distance
+                        field over superblocks, 2 depth bounds of image tiles)
+  raycast_budget        march rounds of a ray before a second kernel carries it
+                        with 16 lanes (0 never)
   raycast_order         dispatch order of the raycast's tiles (slowest of the
                         previous raycast first, reversed, or the hardware's)
 
@@ -90,20 +93,23 @@ def tune_integrate(grid, depths, poses, *, wg_per_cu=INTEGRATE_WG_PER_CU):
     return {"integrate_wg_per_cu": wg}, table
 
 
-def tune_raycast(grid, poses, *, xcd_map=(0, 1), superblocks=(0, 1), order=(0, 1, 2)):
-    """Picks raycast_xcd_map, raycast_superblocks and raycast_order for
-    ``grid.raycast``.  Returns (best tuning dict, [(xcd_map, superblocks,
-    order, ms), ...])."""
+def tune_raycast(grid, poses, *, xcd_map=(0,), skip=(0, 1, 2), order=(0, 2), budget=(0, 12, 16)):
+    """Picks raycast_xcd_map, raycast_skip, raycast_order and raycast_budget
+    for ``grid.raycast`` (the group march replays no superblock field, so
+    skip 1 runs without a budget).  Returns (best tuning dict, [(xcd_map,
+    skip, order, budget, ms), ...])."""
     table = []
-    for xm, sb, od in itertools.product(xcd_map, superblocks, order):
-        grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb, raycast_order=od)
+    for xm, sk, od, bu in itertools.product(xcd_map, skip, order, budget):
+        if sk == 1 and bu:
+            continue
+        grid.set_tuning(raycast_xcd_map=xm, raycast_skip=sk, raycast_order=od, raycast_budget=bu)
 
         def run():
             for E in poses:
                 grid.raycast(E)
 
         run()  # warm (and, for order 2, the costs the first timed raycast orders by)
-        table.append((xm, sb, od, _timed(run)))
-    xm, sb, od, _ = min(table, key=lambda r: r[3])
-    grid.set_tuning(raycast_xcd_map=xm, raycast_superblocks=sb, raycast_order=od)
-    return {"raycast_xcd_map": xm, "raycast_superblocks": sb, "raycast_order": od}, table
+        table.append((xm, sk, od, bu, _timed(run)))
+    xm, sk, od, bu, _ = min(table, key=lambda r: r[4])
+    grid.set_tuning(raycast_xcd_map=xm, raycast_skip=sk, raycast_order=od, raycast_budget=bu)
+    return {"raycast_xcd_map": xm, "raycast_skip": sk, "raycast_order": od, "raycast_budget": bu}, table

@@ -29,6 +29,7 @@
 #include "../../include/tsdf.h"
 #include "tsdf_math.hpp"
 #include "tsdf_march.hpp"
+#include "tsdf_march_group.hpp"
 
 #pragma clang fp contract(off)
 
@@ -1003,13 +1004,74 @@ constexpr int kRayWaves = TSDF_RAY_WAVES;  // 1, 2 or 4
 constexpr int kRayWx = kRayWaves >= 2 ? 2 : 1, kRayWy = kRayWaves >= 4 ? 2 : 1;
 constexpr int kRayTileW = 8 * kRayWx, kRayTileH = 8 * kRayWy;
 
+// The outputs of one ray at pixel o: the world point, its colour, its
+// camera depth (0 unless a crossing) and its UB flag.
+__device__ __forceinline__ void write_ray(const RayResult& r, size_t o, const Pose& pose, float* points, float* colors,
+                                          float* depth, uint8_t* ub)
+{
+    points[o * 3 + 0] = r.p.x;
+    points[o * 3 + 1] = r.p.y;
+    points[o * 3 + 2] = r.p.z;
+    if (colors) {
+        colors[o * 3 + 0] = r.c.x;
+        colors[o * 3 + 1] = r.c.y;
+        colors[o * 3 + 2] = r.c.z;
+    }
+    if (depth)
+        depth[o] = r.code == 1 ? (r.p.x * pose.Ei[8] + r.p.y * pose.Ei[9] + r.p.z * pose.Ei[10] + pose.Ei[11]) : 0.0f;
+    if (ub)
+        ub[o] = (uint8_t)(r.code < 0 ? -r.code : 0);
+}
+
+// k_raycast_tail: the rays k_raycast deferred (queue[0, ray_q[qpar])),
+// kRayGroup lanes per ray (march_group).  Group k of the grid takes rays k,
+// k + groups, ... (a shared counter would put every group's atomic on one
+// address).  The first thread clears the other parity's count for the next
+// raycast.
+constexpr int kRayGroup = 16;
+
+template <typename T, bool kColor>
+__global__ __launch_bounds__(kThreads) void k_raycast_tail(GridView g, Pose pose, RayParams rp, float* points,
+                                                           float* colors, float* depth, uint8_t* ub,
+                                                           const RayState* queue, int qpar)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        g.ctrl->ray_q[qpar ^ 1] = 0u;
+    const Group<kRayGroup> gr;
+    const unsigned int n = g.ctrl->ray_q[qpar];
+    const unsigned int groups = gridDim.x * (kThreads / kRayGroup);
+    unsigned int hit = 0, bad = 0, samples = 0, probes = 0;
+    for (unsigned int e = blockIdx.x * (kThreads / kRayGroup) + threadIdx.x / kRayGroup; e < n; e += groups) {
+        const RayState st = queue[e];
+        const RayResult r = march_group<T, kColor, kRayGroup>(g, pose, rp, st, gr);
+        if (gr.t == 0) {
+            write_ray(r, st.pix, pose, points, colors, depth, ub);
+            hit += r.code == 1;
+            bad += r.code < 0;
+            samples += r.samples;
+            probes += r.probes;
+        }
+    }
+    hit = wave_sum(hit);
+    bad = wave_sum(bad);
+    samples = wave_sum(samples);
+    probes = wave_sum(probes);
+    if (__lane_id() == 0) {
+        const unsigned int who = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+        if (hit) atomicAdd(stat_lane(g, kStRayHits, who), (unsigned long long)hit);
+        if (bad) atomicAdd(stat_lane(g, kStRayUb, who), (unsigned long long)bad);
+        if (samples) atomicAdd(stat_lane(g, kStRaySamples, who), (unsigned long long)samples);
+        if (probes) atomicAdd(stat_lane(g, kStRayProbes, who), (unsigned long long)probes);
+    }
+}
+
 template <typename T, bool kColor, bool kStats>
 __global__ __launch_bounds__(64 * kRayWaves)
 #ifdef TSDF_RAY_WAVES_PER_EU
 __attribute__((amdgpu_waves_per_eu(TSDF_RAY_WAVES_PER_EU, TSDF_RAY_WAVES_PER_EU)))
 #endif
 void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors, float* depth, uint8_t* ub,
-               Footprint fp)
+               Footprint fp, TileBounds tb, RayState* queue, int qpar)
 {
     // the wavefront's start on the 100 MHz constant clock (its duration
     // orders the next raycast's tiles)
@@ -1037,7 +1099,8 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     RayParams rq = rp;
     if (kStats) {
         rq.use_sb = 0;  // the footprint must see every probed key
-    } else if (rq.use_sb) {
+        rq.use_tb = 0;
+    } else if (rq.use_sb || rq.use_tb) {
         const int reach = (int)(rp.dmax / rp.vbl) + 3;
         const float* Ep = pose.E;
         int c[3] = {vb_index(Ep[3], rp.vbl), vb_index(Ep[7], rp.vbl), vb_index(Ep[11], rp.vbl)};
@@ -1048,9 +1111,11 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
             bmin[a] = g.ctrl->bbox_min[a];
             bmax[a] = g.ctrl->bbox_max[a];
         }
-        rq.use_sb = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
+        const int cert = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
+        rq.use_sb = rq.use_sb && cert;
+        rq.use_tb = rq.use_tb && cert;
         // and a distance field of the current superblock bits
-        const SbdfInfo inf = *g.sbdf_info;
+        const SbdfInfo inf = rq.use_sb ? *g.sbdf_info : SbdfInfo{};
         if (rq.use_sb && inf.valid && inf.version == g.ctrl->sb_version) {
             rq.df = g.sbdf;
             for (int a = 0; a < 3; ++a) {
@@ -1062,26 +1127,35 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
         }
     }
     unsigned int hit = 0, bad = 0, samples = 0, probes = 0;
+    // the depth bound of this wavefront's 8 x 8 pixels (one tile of the map)
+    float zt = -1.0f;
+    if (rq.use_tb && tile < n_tiles)
+        zt = tb_depth(tb.z[(size_t)(i / kTbTile) * tb.tiles_x + (j / kTbTile)], tb.epoch);
+    RayState st;
+    bool defer = false;
     if (tile < n_tiles && i < rp.H && j < rp.W) {
-        const RayResult r = march<T, kColor, kStats>(g, pose, rq, i, j, fp);
-        const size_t o = (size_t)i * rp.W + j;
-        points[o * 3 + 0] = r.p.x;
-        points[o * 3 + 1] = r.p.y;
-        points[o * 3 + 2] = r.p.z;
-        if (colors) {
-            colors[o * 3 + 0] = r.c.x;
-            colors[o * 3 + 1] = r.c.y;
-            colors[o * 3 + 2] = r.c.z;
+        const RayResult r = march<T, kColor, kStats>(g, pose, rq, i, j, fp, zt, kStats ? 0 : rq.budget, &st);
+        defer = r.code == kRayDeferred;
+        if (!defer) {
+            write_ray(r, (size_t)i * rp.W + j, pose, points, colors, depth, ub);
+            hit = r.code == 1;
+            bad = r.code < 0;
+            samples = r.samples;
+            probes = r.probes;
         }
-        if (depth)
-            depth[o] = r.code == 1 ? (r.p.x * pose.Ei[8] + r.p.y * pose.Ei[9] + r.p.z * pose.Ei[10] + pose.Ei[11])
-                                   : 0.0f;
-        if (ub)
-            ub[o] = (uint8_t)(r.code < 0 ? -r.code : 0);
-        hit = r.code == 1;
-        bad = r.code < 0;
-        samples = r.samples;
-        probes = r.probes;
+    }
+    // deferred rays: one append per wavefront; k_raycast_tail finishes them
+    if (!kStats) {
+        const unsigned long long dm = __ballot(defer);
+        if (dm) {
+            unsigned int base = 0;
+            const int leader = __ffsll((long long)dm) - 1;
+            if (lane == leader)
+                base = atomicAdd(&g.ctrl->ray_q[qpar], (unsigned int)__popcll(dm));
+            base = __shfl(base, leader);
+            if (defer)
+                queue[base + __popcll(dm & ((1ull << lane) - 1ull))] = st;
+        }
     }
     hit = wave_sum(hit);
     bad = wave_sum(bad);
@@ -1099,6 +1173,73 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
         if (bad) atomicAdd(stat_lane(g, kStRayUb, who), (unsigned long long)bad);
         if (samples) atomicAdd(stat_lane(g, kStRaySamples, who), (unsigned long long)samples);
         if (probes) atomicAdd(stat_lane(g, kStRayProbes, who), (unsigned long long)probes);
+    }
+}
+
+// k_ray_bounds: the tile depth bounds of one raycast (TileBounds in
+// tsdf_march.hpp).  A workgroup takes a contiguous run of brick ids (bricks
+// allocated together are spatial neighbours, so a run covers few tiles),
+// lowers the bounds of the tiles each grown brick may cover in a copy of the
+// tile map in LDS, then merges its copy into the global map with one atomic
+// per tile it touched.  Larger images than kTbLdsTiles tiles lower the global
+// map directly.
+constexpr int kTbLdsTiles = 16384;  // 64 KiB of LDS: 1280 x 720 has 14 400 tiles
+#ifndef TSDF_TB_PER
+#define TSDF_TB_PER 128  // least bricks per workgroup run
+#endif
+#ifndef TSDF_TB_GLOBAL
+#define TSDF_TB_GLOBAL 0
+#endif
+constexpr uint32_t kTbNone = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t f32_bits(float f)
+{
+    return __float_as_uint(f);
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(kThreads) void k_ray_bounds(GridView g, Pose P, TileBounds tb)
+{
+    extern __shared__ uint32_t lz[];  // [tiles] positive depths compare as their bits
+    // the allocated count is known only here: runs of at least TSDF_TB_PER
+    // bricks, the workgroups past the last run leave at once
+    const unsigned int n = min(g.ctrl->alloc_top, g.n_bricks);
+    const unsigned int per = max((n + gridDim.x - 1) / gridDim.x, (unsigned int)TSDF_TB_PER);
+    const unsigned int b0 = blockIdx.x * per, b1 = min(n, b0 + per);
+    if (b0 >= n)
+        return;
+    const int nt = tb.tiles_x * tb.tiles_y;
+    if (kLds) {
+        for (int t = threadIdx.x; t < nt; t += kThreads)
+            lz[t] = kTbNone;
+        __syncthreads();
+    }
+    for (unsigned int id = b0 + threadIdx.x; id < b1; id += kThreads) {
+        const int4 c = g.coords[id];
+        float zmin;
+        int r0, r1, c0, c1;
+        if (!brick_bounds(c.x, c.y, c.z, g.vbl, P, g.cam, g.H, g.W, tb.grow, tb.z_lo, zmin, r0, r1, c0, c1))
+            continue;
+        const uint32_t zb = f32_bits(zmin);
+        for (int ty = r0 / kTbTile; ty <= r1 / kTbTile; ++ty)
+            for (int tx = c0 / kTbTile; tx <= c1 / kTbTile; ++tx) {
+                if (kLds) {
+                    uint32_t* q = &lz[ty * tb.tiles_x + tx];
+                    if (*q > zb)  // most bricks lie behind a nearer one of the tile
+                        atomicMin(q, zb);
+                }
+                else
+                    atomicMin(tb.z + (size_t)ty * tb.tiles_x + tx, tb_pack(tb.epoch, zmin));
+            }
+    }
+    if (kLds) {
+        __syncthreads();
+        const unsigned long long hi = (unsigned long long)(~tb.epoch) << 32;
+        for (int t = threadIdx.x; t < nt; t += kThreads) {
+            const uint32_t v = lz[t];
+            if (v != kTbNone)
+                atomicMin(tb.z + t, hi | v);
+        }
     }
 }
 
@@ -1243,6 +1384,10 @@ struct tsdf_grid {
     Ctrl* ctrl_host;  // pinned
     unsigned long long* stat;       // statistics counters, kStatWords
     unsigned long long* stat_host;  // pinned
+    unsigned long long* tb_z;       // [8x8-pixel tiles] depth bounds of the raycasts (TileBounds)
+    RayState* ray_queue;            // [H * W] rays a raycast deferred to its tail kernel
+    uint32_t ray_qepoch;            // raycasts that used the queue (their parity picks the counters)
+    uint32_t tb_epoch;
     uint32_t* ray_cost;             // [raycast tiles * waves] work of the last raycast's wavefronts
     uint32_t* ray_order;            // [raycast tiles] slowest-first order built from it
     bool ray_order_valid;
@@ -1500,6 +1645,40 @@ struct ProfScope {
     }
 };
 
+// The tile-bounds view of one raycast.  `grow` bounds the distance between a
+// march position and the exact point of its ray parameter (the float
+// position rv drifts by at most half a float spacing per update and
+// component, the ray length l likewise, the step vectors carry their own
+// rounding), with the number of position updates of one march bounded by
+// 4 (d_max + vbl) / min(vl, d_trunc) + 64: every outer round of the march
+// advances l by at least min(vl, d_trunc) net, and every position update but
+// one back-off per round advances it.
+TileBounds tile_bounds(tsdf_grid* g, const Pose& P)
+{
+    TileBounds tb;
+    tb.z = g->tb_z;
+    tb.tiles_x = (g->p.width + kTbTile - 1) / kTbTile;
+    tb.tiles_y = (g->p.height + kTbTile - 1) / kTbTile;
+    tb.epoch = g->tb_epoch;
+    const double vl = g->p.voxel_length, dt = g->p.truncation, vbl = g->vbl, dmax = g->p.max_ray_length;
+    const double n_upd = 4.0 * (dmax + vbl) / (vl < dt ? vl : dt) + 64.0;
+    double tmax = 0.0;
+    for (int a = 0; a < 3; ++a)
+        tmax = std::fmax(tmax, std::fabs((double)P.E[4 * a + 3]));
+    auto spacing = [](double x) { return std::ldexp(1.0, std::ilogb(x) - 23); };  // float spacing at |x|
+    const double R = tmax + dmax + vbl + 1.0;
+    const double drift = std::sqrt(3.0) * n_upd * spacing(R) / 2.0 + n_upd * spacing(dmax + vbl) / 2.0 +
+                         std::sqrt(3.0) * n_upd * spacing(vbl) / 2.0 + std::sqrt(3.0) * dmax * spacing(R) +
+                         8.0 * spacing(R) + 1e-4;
+    tb.grow = (float)drift;
+    // the least depth per unit ray length over the image (its corners)
+    const double ex = std::fmax(std::fabs((0.0 - g->cam.cj) / g->cam.fx), std::fabs((g->p.width - 1.0 - g->cam.cj) / g->cam.fx));
+    const double ey = std::fmax(std::fabs((0.0 - g->cam.ci) / g->cam.fy), std::fabs((g->p.height - 1.0 - g->cam.ci) / g->cam.fy));
+    const double nz_min = 1.0 / std::sqrt(1.0 + ex * ex + ey * ey);
+    tb.z_lo = (float)(0.1 * nz_min * (1.0 - 1e-4) - drift);
+    return tb;
+}
+
 template <bool kStats>
 void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, float* depth, uint8_t* ub,
                     const Footprint& fp, hipStream_t s)
@@ -1520,7 +1699,13 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
     rp.tile_cost = (!kStats && rp.order == 2) ? g->ray_cost : nullptr;
     if (rp.order == 2 && !g->ray_order_valid)
         rp.order = 0;  // no previous raycast to order by yet
-    rp.use_sb = g->tune.raycast_superblocks;
+    rp.use_sb = g->tune.raycast_skip == 1;
+    rp.use_tb = g->tune.raycast_skip == 2 && !kStats;
+    // the group march replays no superblock distance field
+    rp.budget = (kStats || rp.use_sb) ? 0 : g->tune.raycast_budget;
+    const int qpar = (int)(g->ray_qepoch & 1u);
+    if (rp.budget > 0)
+        ++g->ray_qepoch;
     rp.df = nullptr;
     for (int a = 0; a < 3; ++a)
         rp.df_lo[a] = rp.df_dim[a] = 0;
@@ -1530,6 +1715,18 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
         hipLaunchKernelGGL(k_sbdf, dim3(1), dim3(kSbdfThreads), 0, s, view(g), g->sbdf_tmp);
         g->sbdf_gen = g->map_gen;
     }
+    TileBounds tb = tile_bounds(g, P);
+    if (!(tb.z_lo > 1e-3f))
+        rp.use_tb = 0;  // the drift bound leaves no depth to clip at
+    if (rp.use_tb) {
+        tb.epoch = ++g->tb_epoch;
+        const int nt = tb.tiles_x * tb.tiles_y;
+        if (nt <= kTbLdsTiles && !TSDF_TB_GLOBAL)
+            hipLaunchKernelGGL((k_ray_bounds<true>), dim3(g->n_cu * 4), dim3(kThreads), (size_t)nt * sizeof(uint32_t),
+                               s, view(g), P, tb);
+        else
+            hipLaunchKernelGGL((k_ray_bounds<false>), dim3(g->n_cu * 4), dim3(kThreads), 0, s, view(g), P, tb);
+    }
     const int nt = ((rp.W + kRayTileW - 1) / kRayTileW) * ((rp.H + kRayTileH - 1) / kRayTileH);
     const dim3 grid(((nt + 7) / 8) * 8);
     const dim3 block(64 * kRayWaves);
@@ -1537,17 +1734,37 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
     if (g->p.storage == TSDF_STORAGE_F16) {
         if (color)
             hipLaunchKernelGGL((k_raycast<__half, true, kStats>), grid, block, 0, s, view(g), P, rp, points,
-                               colors, depth, ub, fp);
+                               colors, depth, ub, fp, tb, g->ray_queue, qpar);
         else
             hipLaunchKernelGGL((k_raycast<__half, false, kStats>), grid, block, 0, s, view(g), P, rp,
-                               points, colors, depth, ub, fp);
+                               points, colors, depth, ub, fp, tb, g->ray_queue, qpar);
     } else {
         if (color)
             hipLaunchKernelGGL((k_raycast<float, true, kStats>), grid, block, 0, s, view(g), P, rp, points,
-                               colors, depth, ub, fp);
+                               colors, depth, ub, fp, tb, g->ray_queue, qpar);
         else
             hipLaunchKernelGGL((k_raycast<float, false, kStats>), grid, block, 0, s, view(g), P, rp, points,
-                               colors, depth, ub, fp);
+                               colors, depth, ub, fp, tb, g->ray_queue, qpar);
+    }
+    if (rp.budget > 0) {
+        // the deferred rays, kRayGroup lanes each (the queue length is read
+        // on the device; groups past its end leave at once)
+        const dim3 tgrid(g->n_cu * 4);
+        if (g->p.storage == TSDF_STORAGE_F16) {
+            if (color)
+                hipLaunchKernelGGL((k_raycast_tail<__half, true>), tgrid, dim3(kThreads), 0, s, view(g), P, rp,
+                                   points, colors, depth, ub, g->ray_queue, qpar);
+            else
+                hipLaunchKernelGGL((k_raycast_tail<__half, false>), tgrid, dim3(kThreads), 0, s, view(g), P, rp,
+                                   points, colors, depth, ub, g->ray_queue, qpar);
+        } else {
+            if (color)
+                hipLaunchKernelGGL((k_raycast_tail<float, true>), tgrid, dim3(kThreads), 0, s, view(g), P, rp,
+                                   points, colors, depth, ub, g->ray_queue, qpar);
+            else
+                hipLaunchKernelGGL((k_raycast_tail<float, false>), tgrid, dim3(kThreads), 0, s, view(g), P, rp,
+                                   points, colors, depth, ub, g->ray_queue, qpar);
+        }
     }
     // the slowest-first order, rebuilt every kOrderEvery raycasts from the
     // last one's times (consecutive poses see nearly the same rays)
@@ -1698,6 +1915,15 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
         const size_t tiles = (size_t)((p->width + kRayTileW - 1) / kRayTileW) * ((p->height + kRayTileH - 1) / kRayTileH);
         alloc((void**)&g->ray_cost, tiles * kRayWaves * sizeof(uint32_t));
         alloc((void**)&g->ray_order, tiles * sizeof(uint32_t));
+        const size_t t8 = (size_t)((p->width + kTbTile - 1) / kTbTile) * ((p->height + kTbTile - 1) / kTbTile);
+        alloc((void**)&g->tb_z, t8 * sizeof(unsigned long long));
+        alloc((void**)&g->ray_queue, (size_t)p->width * p->height * sizeof(RayState));
+        if (st == TSDF_OK) {
+            // no entry carries an epoch yet (epochs start at 1, ~1 != ~0)
+            hipError_t e = hipMemset(g->tb_z, 0xFF, t8 * sizeof(unsigned long long));
+            if (e != hipSuccess)
+                st = fail(TSDF_HIP_ERROR, std::string("hipMemset: ") + hipGetErrorString(e));
+        }
     }
     if (st == TSDF_OK) {
         // no distance field yet; counters start from zero
@@ -1756,6 +1982,8 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->stat);
     (void)hipFree(g->ray_cost);
     (void)hipFree(g->ray_order);
+    (void)hipFree(g->tb_z);
+    (void)hipFree(g->ray_queue);
     (void)hipFree(g->op_mask);
     (void)hipFree(g->ptmap_batch);
     if (g->batch_exec)
@@ -2120,7 +2348,8 @@ void tsdf_default_tuning(tsdf_tuning* t)
     if (!t)
         return;
     t->raycast_xcd_map = 0;
-    t->raycast_superblocks = 0;
+    t->raycast_skip = 2;
+    t->raycast_budget = 12;
     t->integrate_wg_per_cu = 6;
     t->batch_ops_per_pass = 8;
     t->batch_graph = 1;
@@ -2131,12 +2360,12 @@ tsdf_status tsdf_set_tuning(tsdf_grid* g, const tsdf_tuning* t)
 {
     if (!g || !t)
         return fail(TSDF_BAD_ARG, "NULL grid or tuning");
-    if (t->raycast_xcd_map < 0 || t->raycast_xcd_map > 1 || t->raycast_superblocks < 0 ||
-        t->raycast_superblocks > 1 || t->integrate_wg_per_cu < 1 ||
+    if (t->raycast_xcd_map < 0 || t->raycast_xcd_map > 1 || t->raycast_skip < 0 || t->raycast_skip > 2 ||
+        t->raycast_budget < 0 || t->raycast_budget > 4096 || t->integrate_wg_per_cu < 1 ||
         t->integrate_wg_per_cu > 32 || t->batch_ops_per_pass < 1 || t->batch_ops_per_pass > kMaxOps ||
         t->batch_graph < 0 || t->batch_graph > 1 || t->raycast_order < 0 || t->raycast_order > 2)
-        return fail(TSDF_BAD_ARG, "tuning field out of range (workgroups per CU in [1, 32], raycast order 0, 1 "
-                                  "or 2, flags 0 or 1)");
+        return fail(TSDF_BAD_ARG, "tuning field out of range (workgroups per CU in [1, 32], raycast order and "
+                                  "empty-space skip 0, 1 or 2, flags 0 or 1)");
     g->tune = *t;
     return TSDF_OK;
 }

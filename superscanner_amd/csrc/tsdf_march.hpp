@@ -26,6 +26,7 @@ struct Ctrl {
     int bbox_max[3];
     unsigned int sb_version;  // bumped when a superblock bit is set or the bits are cleared
     unsigned int pad_;
+    unsigned int ray_q[2];    // deferred rays of a raycast, by raycast parity
 };
 
 // Statistics counters that many workgroups add to live outside Ctrl, each
@@ -257,7 +258,168 @@ struct RayParams {
     int use_sb;       // empty-space replay allowed (alias certificate and distance field hold)
     const uint8_t* df;  // the distance field and its box (filled in by the kernel)
     int df_lo[3], df_dim[3];
+    int use_tb;       // tile depth bounds replay allowed (alias certificate holds)
+    int budget;       // rounds of the per-lane march before a ray is deferred (0: never)
 };
+
+// ---------------------------------------------------------------------------
+// Tile depth bounds: the raycast's exact empty-space skip.
+//
+// Before a raycast, every allocated brick's box, grown by `grow` metres, is
+// projected with the raycast's pose; each 8 x 8-pixel tile keeps the least
+// camera depth of the grown boxes whose projection (grown by one pixel)
+// touches it.  For a ray (i, j) of that tile and a march position whose ray
+// parameter is l:
+//  - the march's float position rv stays within `grow` of the exact point
+//    P(l) = t + l dir (its float drift, bounded on the host from the number
+//    of position updates a march can make and the float spacing at its
+//    largest coordinate), so a brick whose box contains rv has a grown box
+//    that contains P(l);
+//  - P(l) projects to pixel (i, j) exactly and has camera depth l (dir . Ez)
+//    = l n_z; a grown box in front of the camera containing P(l) projects
+//    over (i, j), so it counted for the tile, with its least depth <= l n_z;
+// hence l n_z below the tile's bound (less a float margin) means no brick
+// holds rv, and the reference's lookup there misses (with the alias
+// certificate: no other block's key equals rv's).  The march replays those
+// steps without loading: the same positions, lengths and probe counts.
+// Probe-run positions keep l >= 0.1 (the march starts there; a back-off
+// returns less than the skip steps before it), so their depth is at least
+// z_lo = 0.1 min(n_z) - grow: a brick is clipped there, and a brick that
+// crosses that plane is bounded through its box in camera coordinates.  The
+// map needs no clearing: an entry is (~epoch << 32) | bits(depth) under
+// atomicMin, so this raycast's entries are below every older one and a tile
+// without one reads as empty (every lookup of its rays misses).
+// ---------------------------------------------------------------------------
+constexpr int kTbTile = 8;  // tile side in pixels: the 8 x 8 pixels of a raycast wavefront
+
+struct TileBounds {
+    unsigned long long* z;  // [tiles_y * tiles_x]
+    int tiles_x, tiles_y;
+    uint32_t epoch;
+    float grow;  // metres a brick box grows by
+    float z_lo;  // least depth of a probe-run position
+};
+
+__host__ __device__ __forceinline__ unsigned long long tb_pack(uint32_t epoch, float z)
+{
+    union {
+        float f;
+        uint32_t u;
+    } b;
+    b.f = z;
+    return ((unsigned long long)(~epoch) << 32) | b.u;
+}
+
+// The tile bound of this raycast (epoch), +inf for a tile no brick touched.
+__host__ __device__ __forceinline__ float tb_depth(unsigned long long e, uint32_t epoch)
+{
+    union {
+        float f;
+        uint32_t u;
+    } b;
+    b.u = (uint32_t)e;
+    return (uint32_t)(e >> 32) == ~epoch ? b.f : __builtin_inff();
+}
+
+// 1 / z to within a few ulps (the hardware reciprocal on the GPU); used only
+// where a pixel of margin absorbs its error.
+__host__ __device__ __forceinline__ float approx_rcp(float z)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(z);
+#else
+    return 1.0f / z;
+#endif
+}
+
+// Bounds of brick (bx, by, bz): false when no probe-run position can lie in
+// it (all of it nearer than z_lo, or outside the image), else its least
+// depth beyond z_lo and the inclusive pixel range [r0, r1] x [c0, c1] its
+// grown box may cover.
+__host__ __device__ __forceinline__ bool brick_bounds(int bx, int by, int bz, float vbl, const Pose& P,
+                                                      const Camera& cam, int H, int W, float grow, float z_lo,
+                                                      float& zmin, int& r0, int& r1, int& c0, int& c1)
+{
+    const float lo[3] = {vbl * (float)bx - grow, vbl * (float)by - grow, vbl * (float)bz - grow};
+    const float hi[3] = {vbl * (float)bx + vbl + grow, vbl * (float)by + vbl + grow, vbl * (float)bz + vbl + grow};
+    float cx[8], cy[8], cz[8];
+    float x0 = __builtin_inff(), x1 = -__builtin_inff(), y0 = __builtin_inff(), y1 = -__builtin_inff();
+    float z0 = __builtin_inff(), z1 = -__builtin_inff();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float x = (k & 1) ? hi[0] : lo[0], y = (k & 2) ? hi[1] : lo[1], z = (k & 4) ? hi[2] : lo[2];
+        cx[k] = x * P.Ei[0] + y * P.Ei[1] + z * P.Ei[2] + P.Ei[3];
+        cy[k] = x * P.Ei[4] + y * P.Ei[5] + z * P.Ei[6] + P.Ei[7];
+        cz[k] = x * P.Ei[8] + y * P.Ei[9] + z * P.Ei[10] + P.Ei[11];
+        x0 = fminf(x0, cx[k]);
+        x1 = fmaxf(x1, cx[k]);
+        y0 = fminf(y0, cy[k]);
+        y1 = fmaxf(y1, cy[k]);
+        z0 = fminf(z0, cz[k]);
+        z1 = fmaxf(z1, cz[k]);
+    }
+    if (!(z1 >= z_lo))
+        return false;
+    float rmin, rmax, cmin, cmax;
+    if (z0 >= z_lo) {
+        // in front of the clip plane: the projection is the convex hull of
+        // the corners' projections
+        rmin = cmin = __builtin_inff();
+        rmax = cmax = -__builtin_inff();
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            // a reciprocal within a few ulps: far inside the pixel of margin
+            const float iz = approx_rcp(cz[k]);
+            const float r = cy[k] * cam.fy * iz + cam.ci, c = cx[k] * cam.fx * iz + cam.cj;
+            rmin = fminf(rmin, r);
+            rmax = fmaxf(rmax, r);
+            cmin = fminf(cmin, c);
+            cmax = fmaxf(cmax, c);
+        }
+    } else {
+        // crossing it: the part at depth [z_lo, z1] lies in the box
+        // [x0, x1] x [y0, y1] x [z_lo, z1], over which x / z and y / z take
+        // their extremes at the corners
+        z0 = z_lo;
+        const float i0 = approx_rcp(z0), i1 = approx_rcp(z1);
+        const float u0 = fminf(x0 * i0, x0 * i1), u1 = fmaxf(x1 * i0, x1 * i1);
+        const float v0 = fminf(y0 * i0, y0 * i1), v1 = fmaxf(y1 * i0, y1 * i1);
+        cmin = fminf(u0 * cam.fx, u1 * cam.fx) + cam.cj;
+        cmax = fmaxf(u0 * cam.fx, u1 * cam.fx) + cam.cj;
+        rmin = fminf(v0 * cam.fy, v1 * cam.fy) + cam.ci;
+        rmax = fmaxf(v0 * cam.fy, v1 * cam.fy) + cam.ci;
+    }
+    // one pixel of margin for the rounding of the projections
+    rmin = fmaxf(rmin - 1.0f, -1.0f);
+    cmin = fmaxf(cmin - 1.0f, -1.0f);
+    rmax = fminf(rmax + 1.0f, (float)H);
+    cmax = fminf(cmax + 1.0f, (float)W);
+    if (!(rmin <= rmax) || !(cmin <= cmax))
+        return false;  // off the image (or NaN)
+    r0 = (int)ceilf(rmin);
+    r1 = (int)floorf(rmax);
+    c0 = (int)ceilf(cmin);
+    c1 = (int)floorf(cmax);
+    r0 = r0 < 0 ? 0 : r0;
+    c0 = c0 < 0 ? 0 : c0;
+    r1 = r1 > H - 1 ? H - 1 : r1;
+    c1 = c1 > W - 1 ? W - 1 : c1;
+    if (r0 > r1 || c0 > c1)
+        return false;
+    zmin = z0;
+    return true;
+}
+
+// The ray parameter below which every lookup of ray (i, j) misses: the
+// tile's bound over the ray's depth per unit length, less a float margin;
+// d_max when no brick touched the tile.
+__host__ __device__ __forceinline__ float tb_l_safe(float zt, float nz, float dmax)
+{
+    if (!(zt < __builtin_inff()))
+        return dmax;
+    const float l = (zt * (1.0f - 1e-5f) - 1e-5f) / nz;
+    return l < dmax ? l : dmax;
+}
 
 // The reference's index rule on a quotient q = RN32(a / d) (exact through
 // quot_rn): (int)q for a >= 0, (int)(q - 1) below.  A position on a ray lies
@@ -290,9 +452,26 @@ __host__ __device__ __forceinline__ void vin3(f3 r, int bx, int by, int bz, cons
 
 struct RayResult {
     f3 p, c;
-    int code;  // 1 zero crossing, 0 miss, -TSDF_RAY_UB_* reference UB
+    int code;  // 1 zero crossing, 0 miss, -TSDF_RAY_UB_* reference UB, kRayDeferred
     unsigned int samples, probes;
 };
+
+// A march stopped at the top of its outer loop (code kRayDeferred) and
+// resumed later by march_group: its position, length, the last sample's
+// weight sign, the lookup loaded ahead for the next position and the counts
+// so far.  Everything else of the march is recomputed from the pixel.
+constexpr int kRayDeferred = 2;
+struct RayState {
+    float rx, ry, rz, l;
+    uint32_t pix;          // i * W + j
+    uint32_t flags;        // bit 0 Lp, bit 1 pre_valid
+    uint32_t pre_key, pre_bk;
+    unsigned long long pre_slot;
+    uint32_t samples, probes;
+    float zt;              // the tile depth bound of the ray (< 0: none)
+    uint32_t pad[3];
+};
+static_assert(sizeof(RayState) == 64, "one 64-byte record per deferred ray");
 
 constexpr int kRayUbVoxelIndex = 1;  // TSDF_RAY_UB_VOXEL_INDEX
 constexpr int kRayUbMissingOut = 2;  // TSDF_RAY_UB_MISSING_OUT
@@ -451,7 +630,8 @@ __host__ __device__ __forceinline__ f3 sel3(bool c, f3 a, f3 b) { return {c ? a.
 template <bool kStats>
 __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayParams& rp, const Footprint& fp, f3& rv,
                                                   float& l, const f3 delta, float dl, int& steps, unsigned int& probes,
-                                                  bool pre_valid = false, uint32_t pre_key = 0, uint32_t pre_bk = 0,
+                                                  float l_safe, int& rounds, bool pre_valid = false,
+                                                  uint32_t pre_key = 0, uint32_t pre_bk = 0,
                                                   unsigned long long pre_slot = 0)
 {
     const float dmax = rp.dmax;
@@ -470,6 +650,23 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
     bool have = false;  // nrep holds the replay count for the block of rv
     int nrep = 0;
     while (first || (id < 0 && l < dmax)) {
+        // tile depth bounds: the lookups at ray parameters below l_safe
+        // miss (l_safe <= d_max, so every such miss is followed by a step)
+        if (first && l < l_safe) {
+            ++probes;
+            first = false;
+        }
+        if (!first) {
+            for (;;) {
+                const float ln = l + dl;
+                if (!(ln < l_safe))
+                    break;
+                rv = add3(rv, delta);
+                l = ln;
+                ++steps;
+                ++probes;
+            }
+        }
         if (rp.use_sb && !first) {
             if (!have) {
                 int cx, cy, cz;
@@ -507,6 +704,7 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
             }
         }
         bool go = true, pend = false;
+        ++rounds;
         uint32_t pkey = 0, pbk = 0;
 #pragma unroll
         for (int k = 0; k < kSkipLook; ++k) {
@@ -537,9 +735,87 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
     return id;
 }
 
+// The zero crossing, GPUVoxelGrid.cpp:655-689: rv is the first position at
+// or behind the surface, in block (bx, by, bz) = brick cur, with sample
+// (xs, xw) at voxel xv; out gets the point (and colour) or a UB code.
+template <typename T, bool kColor, bool kStats>
+__host__ __device__ __forceinline__ void crossing(const GridView& g, const RayParams& rp, const Footprint& fp, f3 rv,
+                                                  float l, const f3 dir, int bx, int by, int bz, int cur, float xs,
+                                                  float xw, int xv, RayResult& out)
+{
+    const float vl = rp.vl, vbl = rp.vbl;
+    const int ivx = xv & 7, ivy = (xv >> 3) & 7, ivz = xv >> 6;
+    const f3 vm_in = {voxel_mid(ivx, bx, vbl, vl), voxel_mid(ivy, by, vbl, vl), voxel_mid(ivz, bz, vbl, vl)};
+    rv.x -= vl * dir.x;
+    rv.y -= vl * dir.y;
+    rv.z -= vl * dir.z;
+    l -= vl;
+    (void)l;
+    int qx, qy, qz, wx, wy, wz;
+    vb3(rv, rp, qx, qy, qz);
+    vin3(rv, qx, qy, qz, rp, wx, wy, wz);
+    const f3 vm_out = {voxel_mid(wx, qx, vbl, vl), voxel_mid(wy, qy, vbl, vl), voxel_mid(wz, qz, vbl, vl)};
+    int ob = cur;
+    if (qx != bx || qy != by || qz != bz) {
+        ob = probe_count<kStats>(g, fp, qx, qy, qz, out.probes);
+        if (ob < 0) {
+            out.code = -kRayUbMissingOut;
+            return;
+        }
+    }
+    if (!vin_ok(wx, wy, wz)) {
+        out.code = -kRayUbVoxelIndex;
+        return;
+    }
+    const int vo_i = wx + kVPE * wy + kVPE * kVPE * wz;
+    float os, ow;
+    ld_pair((const T*)g.payload + (size_t)ob * (2 * kVPB), vo_i, os, ow);
+    ++out.samples;
+    if (kStats)
+        fp_voxel(fp, ob, vo_i);
+    const float xx = (ow * os / vl + xw * (1.0f - xs / vl)) / (xw + ow);
+    f3 p;
+    p.x = vm_out.x + xx * (vm_in.x - vm_out.x);
+    p.y = vm_out.y + xx * (vm_in.y - vm_out.y);
+    p.z = vm_out.z + xx * (vm_in.z - vm_out.z);
+    const float sc = dot3(p, dir) - dot3(rv, dir);
+    out.p.x = rv.x + sc * dir.x;
+    out.p.y = rv.y + sc * dir.y;
+    out.p.z = rv.z + sc * dir.z;
+    if (kColor) {
+        const float* co = g.rgb + (size_t)ob * (3 * kVPB);
+        const float* cc = g.rgb + (size_t)cur * (3 * kVPB);
+        out.c.x = (float)((0.5 * (double)co[vo_i] + 0.5 * (double)cc[xv]) / 255.0);
+        out.c.y = (float)((0.5 * (double)co[vo_i + kVPB] + 0.5 * (double)cc[xv + kVPB]) / 255.0);
+        out.c.z = (float)((0.5 * (double)co[vo_i + 2 * kVPB] + 0.5 * (double)cc[xv + 2 * kVPB]) / 255.0);
+    }
+    out.code = 1;
+}
+
+// The ray of pixel (i, j), GPUVoxelGrid.cpp:565-569 and 717-722: the unit
+// camera ray n, the start position rv (l = 0.1) and the direction dir.
+__host__ __device__ __forceinline__ void ray_setup(const GridView& g, const Pose& pose, int i, int j, f3& n, f3& rv,
+                                                   f3& dir)
+{
+    n = get_point(g.cam, i, j, 1.0f);
+    const float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+    n.x /= len;
+    n.y /= len;
+    n.z /= len;
+    rv = xform(n, pose.E);
+    dir = {rv.x - pose.E[3], rv.y - pose.E[7], rv.z - pose.E[11]};
+    const float a0 = 1.0f - 0.1f;
+    rv.x -= a0 * dir.x;
+    rv.y -= a0 * dir.y;
+    rv.z -= a0 * dir.z;
+}
+
+// budget > 0: after that many probe rounds and walk windows the march stops
+// at the top of its outer loop, returns kRayDeferred and leaves its state in
+// *st (march_group resumes it).
 template <typename T, bool kColor, bool kStats>
 __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const RayParams& rp, int i, int j,
-                                    const Footprint& fp)
+                                    const Footprint& fp, float zt = -1.0f, int budget = 0, RayState* st = nullptr)
 {
     RayResult out;
     out.p = {0.f, 0.f, 0.f};
@@ -549,18 +825,12 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
     out.probes = 0;
     const float vl = rp.vl, vbl = rp.vbl, dt = rp.dt, dmax = rp.dmax;
     // createNormPcdTag, GPUVoxelGrid.cpp:717-722, and the ray set-up, :565-569
-    f3 n = get_point(g.cam, i, j, 1.0f);
-    const float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
-    n.x /= len;
-    n.y /= len;
-    n.z /= len;
-    f3 rv = xform(n, pose.E);
-    const f3 dir = {rv.x - pose.E[3], rv.y - pose.E[7], rv.z - pose.E[11]};
+    f3 n, rv, dir;
+    ray_setup(g, pose, i, j, n, rv, dir);
+    // the tile bound of this ray's tile (zt < 0: no replay)
+    const float l_safe = (rp.use_tb && zt >= 0.0f) ? tb_l_safe(zt, n.z, dmax) : -1.0f;
     float l = 0.1f;
-    const float a0 = 1.0f - l;
-    rv.x -= a0 * dir.x;
-    rv.y -= a0 * dir.y;
-    rv.z -= a0 * dir.z;
+    int rounds = 0;
     const f3 skip = {vbl * dir.x, vbl * dir.y, vbl * dir.z};  // the reference's vbl * dir
     const f3 step = {vl * dir.x, vl * dir.y, vl * dir.z};     // and vl * dir
 
@@ -570,11 +840,27 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
     uint32_t pre_key = 0, pre_bk = 0;
     unsigned long long pre_slot = 0;
     while (l < dmax) {  // outer loop; done / Fn leave by return
+        if (budget > 0 && rounds >= budget && st) {
+            st->rx = rv.x;
+            st->ry = rv.y;
+            st->rz = rv.z;
+            st->l = l;
+            st->pix = (uint32_t)(i * rp.W + j);
+            st->flags = (Lp ? 1u : 0u) | (pre_valid ? 2u : 0u);
+            st->pre_key = pre_key;
+            st->pre_bk = pre_bk;
+            st->pre_slot = pre_slot;
+            st->samples = out.samples;
+            st->probes = out.probes;
+            st->zt = zt;
+            out.code = kRayDeferred;
+            return out;
+        }
         // ---- the block lookup at rv and the empty-block skip ------------
         // (GPUVoxelGrid.cpp:589-604): probe rv, then steps of vbl * dir
         int count = 0;
-        int id = probe_run<kStats>(g, rp, fp, rv, l, skip, vbl, count, out.probes, pre_valid, pre_key, pre_bk,
-                                   pre_slot);
+        int id = probe_run<kStats>(g, rp, fp, rv, l, skip, vbl, count, out.probes, l_safe, rounds, pre_valid,
+                                   pre_key, pre_bk, pre_slot);
         pre_valid = false;
         if (!(l < dmax))
             break;
@@ -588,7 +874,7 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
             l -= back;
             const f3 dstep = {dt * dir.x, dt * dir.y, dt * dir.z};
             int steps = 0;
-            id = probe_run<kStats>(g, rp, fp, rv, l, dstep, dt, steps, out.probes);
+            id = probe_run<kStats>(g, rp, fp, rv, l, dstep, dt, steps, out.probes, l_safe, rounds);
         }
         if (!(l < dmax))
             break;
@@ -600,6 +886,7 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
         int vx, vy, vz;
         vin3(rv, bx, by, bz, rp, vx, vy, vz);
         while (true) {
+            ++rounds;
             // voxel index of positions rv, rv + vl dir, ...: -2 where the
             // loop test fails (range or block left), -1 where the index is
             // outside [0, 8) (reference UB), else x + 8y + 64z
@@ -698,51 +985,7 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
             if (ending == 3)
                 return out;  // Fn: miss
             // zero crossing, GPUVoxelGrid.cpp:655-689
-            const int ivx = xv & 7, ivy = (xv >> 3) & 7, ivz = xv >> 6;
-            const f3 vm_in = {voxel_mid(ivx, bx, vbl, vl), voxel_mid(ivy, by, vbl, vl), voxel_mid(ivz, bz, vbl, vl)};
-            rv.x -= vl * dir.x;
-            rv.y -= vl * dir.y;
-            rv.z -= vl * dir.z;
-            l -= vl;
-            int qx, qy, qz, wx, wy, wz;
-            vb3(rv, rp, qx, qy, qz);
-            vin3(rv, qx, qy, qz, rp, wx, wy, wz);
-            const f3 vm_out = {voxel_mid(wx, qx, vbl, vl), voxel_mid(wy, qy, vbl, vl), voxel_mid(wz, qz, vbl, vl)};
-            int ob = cur;
-            if (qx != bx || qy != by || qz != bz) {
-                ob = probe_count<kStats>(g, fp, qx, qy, qz, out.probes);
-                if (ob < 0) {
-                    out.code = -kRayUbMissingOut;
-                    return out;
-                }
-            }
-            if (!vin_ok(wx, wy, wz)) {
-                out.code = -kRayUbVoxelIndex;
-                return out;
-            }
-            const int vo_i = wx + kVPE * wy + kVPE * kVPE * wz;
-            float os, ow;
-            ld_pair((const T*)g.payload + (size_t)ob * (2 * kVPB), vo_i, os, ow);
-            ++out.samples;
-            if (kStats)
-                fp_voxel(fp, ob, vo_i);
-            const float xx = (ow * os / vl + xw * (1.0f - xs / vl)) / (xw + ow);
-            f3 p;
-            p.x = vm_out.x + xx * (vm_in.x - vm_out.x);
-            p.y = vm_out.y + xx * (vm_in.y - vm_out.y);
-            p.z = vm_out.z + xx * (vm_in.z - vm_out.z);
-            const float sc = dot3(p, dir) - dot3(rv, dir);
-            out.p.x = rv.x + sc * dir.x;
-            out.p.y = rv.y + sc * dir.y;
-            out.p.z = rv.z + sc * dir.z;
-            if (kColor) {
-                const float* co = g.rgb + (size_t)ob * (3 * kVPB);
-                const float* cc = g.rgb + (size_t)cur * (3 * kVPB);
-                out.c.x = (float)((0.5 * (double)co[vo_i] + 0.5 * (double)cc[xv]) / 255.0);
-                out.c.y = (float)((0.5 * (double)co[vo_i + kVPB] + 0.5 * (double)cc[xv + kVPB]) / 255.0);
-                out.c.z = (float)((0.5 * (double)co[vo_i + 2 * kVPB] + 0.5 * (double)cc[xv + 2 * kVPB]) / 255.0);
-            }
-            out.code = 1;
+            crossing<T, kColor, kStats>(g, rp, fp, rv, l, dir, bx, by, bz, cur, xs, xw, xv, out);
             return out;
         }
     }

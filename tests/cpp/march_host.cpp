@@ -86,8 +86,11 @@ extern "C" int march_host(int H, int W, float fx, float fy, float cx, float cy, 
     rp.H = H;
     rp.W = W;
     rp.xcd_map = 0;
+    const int mode = use_sb;  // empty-space skip: 0 none, 1 superblock distance field, 2 tile depth bounds
+    use_sb = mode == 1;
     rp.use_sb = 0;
-    if (use_sb) {  // the certificate the raycast kernel evaluates
+    rp.use_tb = 0;
+    if (mode) {  // the certificate the raycast kernel evaluates
         const int reach = (int)(rp.dmax / rp.vbl) + 3;
         const int c[3] = {vb_index(E[3], rp.vbl), vb_index(E[7], rp.vbl), vb_index(E[11], rp.vbl)};
         int plo[3], phi[3];
@@ -95,7 +98,8 @@ extern "C" int march_host(int H, int W, float fx, float fy, float cx, float cy, 
             plo[a] = c[a] - reach;
             phi[a] = c[a] + reach;
         }
-        rp.use_sb = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
+        rp.use_sb = (use_sb && alias_free(bmin, bmax, plo, phi)) ? 1 : 0;
+        rp.use_tb = (mode == 2 && alias_free(bmin, bmax, plo, phi)) ? 1 : 0;
     }
     std::vector<uint8_t> df, df_tmp;
     rp.df = nullptr;
@@ -113,13 +117,49 @@ extern "C" int march_host(int H, int W, float fx, float fy, float cx, float cy, 
         sbdf_build_host(sb.data(), rp.df_lo, rp.df_dim, df.data(), df_tmp.data());
         rp.df = df.data();
     }
-    *certified = rp.use_sb;
+    // tile depth bounds, as k_ray_bounds builds them (grow as the host side
+    // of tsdf_raycast computes it for this pose)
+    const int tx = (W + kTbTile - 1) / kTbTile, ty = (H + kTbTile - 1) / kTbTile;
+    std::vector<unsigned long long> tbz((size_t)tx * ty, ~0ull);
+    int near = 0;
+    if (rp.use_tb) {
+        const double vl_ = vl, dt_ = trunc, vbl_ = g.vbl, dm = dmax;
+        const double n_upd = 4.0 * (dm + vbl_) / (vl_ < dt_ ? vl_ : dt_) + 64.0;
+        double tmax = 0.0;
+        for (int a = 0; a < 3; ++a)
+            tmax = std::fmax(tmax, std::fabs((double)E[4 * a + 3]));
+        auto spacing = [](double x) { return std::ldexp(1.0, std::ilogb(x) - 23); };
+        const double R = tmax + dm + vbl_ + 1.0;
+        const float grow = (float)(std::sqrt(3.0) * n_upd * spacing(R) / 2.0 + n_upd * spacing(dm + vbl_) / 2.0 +
+                                   std::sqrt(3.0) * n_upd * spacing(vbl_) / 2.0 + std::sqrt(3.0) * dm * spacing(R) +
+                                   8.0 * spacing(R) + 1e-4);
+        const double ex = std::fmax(std::fabs((0.0 - g.cam.cj) / g.cam.fx), std::fabs((W - 1.0 - g.cam.cj) / g.cam.fx));
+        const double ey = std::fmax(std::fabs((0.0 - g.cam.ci) / g.cam.fy), std::fabs((H - 1.0 - g.cam.ci) / g.cam.fy));
+        const float z_lo = (float)(0.1 / std::sqrt(1.0 + ex * ex + ey * ey) * (1.0 - 1e-4) - grow);
+        near = !(z_lo > 1e-3f);
+        for (int b = 0; b < n && !near; ++b) {
+            const int* c = coords + 3 * b;
+            float zmin;
+            int r0, r1, c0, c1;
+            if (!brick_bounds(c[0], c[1], c[2], g.vbl, P, g.cam, H, W, grow, z_lo, zmin, r0, r1, c0, c1))
+                continue;
+            const unsigned long long v = tb_pack(1u, zmin);
+            for (int y = r0 / kTbTile; y <= r1 / kTbTile; ++y)
+                for (int x = c0 / kTbTile; x <= c1 / kTbTile; ++x)
+                    if (tbz[(size_t)y * tx + x] > v)
+                        tbz[(size_t)y * tx + x] = v;
+        }
+        if (near)
+            rp.use_tb = 0;
+    }
+    *certified = mode == 1 ? rp.use_sb : (mode == 2 ? rp.use_tb : 0);
     Footprint fp;
     std::memset(&fp, 0, sizeof(fp));
     uint64_t ns = 0, np = 0;
     for (int i = 0; i < H; ++i)
         for (int j = 0; j < W; ++j) {
-            const RayResult r = march<float, false, false>(g, P, rp, i, j, fp);
+            const float zt = rp.use_tb ? tb_depth(tbz[(size_t)(i / kTbTile) * tx + j / kTbTile], 1u) : -1.0f;
+            const RayResult r = march<float, false, false>(g, P, rp, i, j, fp, zt);
             const size_t o = (size_t)i * W + j;
             points[o * 3 + 0] = r.p.x;
             points[o * 3 + 1] = r.p.y;

@@ -185,7 +185,7 @@ def test_clean_equals_oracle(gpu, color):
     assert g.size() == o.n_bricks()
     assert_grids_equal(g, o, color=color)
     for sb in (0, 1):
-        g.set_tuning(raycast_superblocks=sb)
+        g.set_tuning(raycast_skip=sb)
         assert_raycast_equal(g, o, poses[3], color=color)
     assert g.cleanMemory() == o.clean()
     assert_grids_equal(g, o, color=color)

@@ -99,7 +99,7 @@ def test_plain_small_full_size(gpu):
         assert after["ray_probes"] - before["ray_probes"] == probes, order
     g.set_tuning(raycast_order=2)
     # the empty-space replay keeps every point and count
-    g.set_tuning(raycast_superblocks=1)
+    g.set_tuning(raycast_skip=1)
     before = g.stats()
     samples = probes = 0
     for k in (0, 7, 15, 22):
@@ -109,6 +109,50 @@ def test_plain_small_full_size(gpu):
     after = g.stats()
     assert after["ray_samples"] - before["ray_samples"] == samples
     assert after["ray_probes"] - before["ray_probes"] == probes
+
+
+@pytest.mark.parametrize("budget", [1, 3, 16])
+@pytest.mark.parametrize("skip", [0, 2])
+def test_deferred_rays_equal_oracle(gpu, budget, skip):
+    """Rays the per-lane march hands over after `budget` rounds are finished
+    by the second kernel, 16 lanes per ray (the walk crossing bricks within a
+    round): points, depth, UB flags and the sample and lookup counts equal the
+    oracle's (budget 1 hands over nearly every ray after its first round)."""
+    cfg, intr, poses, depth = plain_small(n=30)
+    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], intr,
+                     n_buckets=cfg["n_buckets"], n_bricks=1 << 19)
+    for k in range(len(poses)):
+        g.integrate(None, dev_u16(depth[k]), poses[k])
+        o.integrate(depth[k], poses[k])
+    g.set_tuning(raycast_budget=budget, raycast_skip=skip)
+    g.sync()
+    for k in (0, 7, 15, 22):
+        before = g.stats()
+        r = assert_raycast_equal(g, o, poses[k])
+        after = g.stats()
+        assert after["ray_samples"] - before["ray_samples"] == r["counts"]["samples"], k
+        assert after["ray_probes"] - before["ray_probes"] == r["counts"]["probes"], k
+        assert after["ray_hits"] - before["ray_hits"] == r["counts"]["n_hit"], k
+        assert after["ray_ub"] - before["ray_ub"] == r["counts"]["n_ub"], k
+
+
+@pytest.mark.parametrize("budget", [1, 8])
+def test_deferred_rays_room_and_colour(gpu, budget):
+    """The same on room frames (848x480 at half size) and with colour."""
+    cfg, intr, poses, depth = room(12, H=240, W=424)
+    rng = np.random.default_rng(5)
+    rgb = rng.integers(0, 256, size=(12, 240, 424, 3), dtype=np.uint8)
+    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], 240, 424, intr, with_color=True, max_weight=128.0)
+    for k in range(12):
+        g.integrate(dev_u8(rgb[k]), dev_u16(depth[k]), poses[k])
+        o.integrate(depth[k], poses[k], 1, rgb[k])
+    g.set_tuning(raycast_budget=budget)
+    for k in (3, 11):
+        before = g.stats()
+        r = assert_raycast_equal(g, o, poses[k], color=True)
+        after = g.stats()
+        assert after["ray_samples"] - before["ray_samples"] == r["counts"]["samples"]
+        assert after["ray_probes"] - before["ray_probes"] == r["counts"]["probes"]
 
 
 def test_deintegrate_sequence(gpu):
@@ -171,7 +215,7 @@ def test_room_large_frames_with_weight_cap(gpu):
     assert_raycast_equal(g, o, poses[5])
     # with the empty-space replay: the same points and the same (reference)
     # probe and sample counts, fewer actual probes
-    g.set_tuning(raycast_superblocks=1)
+    g.set_tuning(raycast_skip=1)
     for k in (2, 5, 7):
         before = g.stats()
         r = assert_raycast_equal(g, o, poses[k])

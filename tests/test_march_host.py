@@ -48,13 +48,13 @@ def run_host(L, o, cfg, intr, E, H, W, use_sb):
                       ex["sdf"].ctypes.data, ex["w"].ctypes.data, C.c_uint32(nb), int(use_sb), E.ctypes.data,
                       pts.ctypes.data, dep.ctypes.data, ub.ctypes.data, C.byref(ns), C.byref(npr), C.byref(cert))
     assert st == 0
-    assert cert.value == int(use_sb)  # these scenes are certified alias-free
+    assert cert.value == int(use_sb > 0)  # these scenes are certified alias-free, no brick near the camera
     return pts, dep, ub, ns.value, npr.value
 
 
 def check(L, o, cfg, intr, E, H, W):
     r = o.raycast(E)
-    for use_sb in (0, 1):
+    for use_sb in (0, 1, 2):  # no skip, superblock distance field, tile depth bounds
         compare(run_host(L, o, cfg, intr, E, H, W, use_sb), r)
     return r
 
