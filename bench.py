@@ -403,33 +403,55 @@ KERNEL_NAMES = {"collect": "k_collect(", "integrate": "k_integrate<", "raycast":
 
 
 def maintenance_leg(grid):
-    """cleanMemory and mesh extraction on the map the timed steps built."""
+    """cleanMemory and mesh extraction on the map the timed steps built:
+    marching cubes (the reference's marchingCubesHost) and the marching
+    tetrahedra option."""
     import torch
     torch.cuda.synchronize()
     out = {"bricks": grid.size()}
+    reps = 5
+    # marching cubes: indexed mesh with normals, device buffers sized once
     t0 = time.perf_counter()
-    m = grid.extract_mesh()
+    m = grid.extract_mesh(method="cubes")
+    out["cubes_first_call_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    V, T = int(m["vertices"].shape[0]), int(m["faces"].shape[0])
+    dev = grid.device
+    verts = torch.empty((V, 3), dtype=torch.float64, device=dev)
+    nrm = torch.empty((V, 3), dtype=torch.float64, device=dev)
+    faces = torch.empty((T, 3), dtype=torch.int32, device=dev)
+    nv, nt = C.c_uint64(0), C.c_uint64(0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        grid._L.tsdf_marching_cubes(grid._h, C.c_void_p(verts.data_ptr()), None, C.c_void_p(nrm.data_ptr()),
+                                    C.c_void_p(faces.data_ptr()), None, V, T, C.byref(nv), C.byref(nt), None)
+    cubes_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["cubes"] = {"triangles": T, "vertices": V, "ms": round(cubes_ms, 3),
+                    "Mtri_per_s": round(T / (cubes_ms * 1e-3) / 1e6, 1),
+                    "note": "tsdf_marching_cubes per call, synchronous: vertices (double), faces, vertex "
+                            "normals in device buffers"}
+    del verts, nrm, faces
+    t0 = time.perf_counter()
+    m = grid.extract_mesh(method="tetrahedra")
     torch.cuda.synchronize()
     out["mesh_first_call_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     n_tri = int(m["triangles"].shape[0])
     cap = n_tri + 1
     tris = torch.empty((cap, 3, 3), dtype=torch.float32, device=grid.device)
     cnt = C.c_uint64(0)
-    reps = 5
     t0 = time.perf_counter()
     for _ in range(reps):
         grid._L.tsdf_extract_mesh(grid._h, C.c_void_p(tris.data_ptr()), None, None, cap, C.byref(cnt), None)
     mesh_ms = (time.perf_counter() - t0) * 1e3 / reps
     t0 = time.perf_counter()
     for _ in range(reps):
-        grid.extract_mesh()
+        grid.extract_mesh(method="tetrahedra")
     indexed_ms = (time.perf_counter() - t0) * 1e3 / reps
-    out.update({"triangles": n_tri, "vertices": int(m["vertices"].shape[0]), "mesh_ms": round(mesh_ms, 3),
-                "mesh_Mtri_per_s": round(n_tri / (mesh_ms * 1e-3) / 1e6, 1),
-                "indexed_mesh_ms": round(indexed_ms, 2),
-                "note": "mesh_ms: tsdf_extract_mesh (triangle soup) per call, synchronous; indexed_mesh_ms: "
-                        "GPUVoxelGrid.extract_mesh end to end (soup + keys, welding and vertex normals on the "
-                        "GPU, copy of the indexed mesh to the host)"})
+    out["tetrahedra"] = {"triangles": n_tri, "vertices": int(m["vertices"].shape[0]), "mesh_ms": round(mesh_ms, 3),
+                         "mesh_Mtri_per_s": round(n_tri / (mesh_ms * 1e-3) / 1e6, 1),
+                         "indexed_mesh_ms": round(indexed_ms, 2),
+                         "note": "mesh_ms: tsdf_extract_mesh (triangle soup) per call, synchronous; "
+                                 "indexed_mesh_ms: GPUVoxelGrid.extract_mesh(method='tetrahedra') end to end (soup + "
+                                 "keys, welding and vertex normals on the GPU, copy of the indexed mesh to the host)"}
     t0 = time.perf_counter()
     removed = grid.cleanMemory()
     out["clean_ms"] = round((time.perf_counter() - t0) * 1e3, 3)

@@ -308,6 +308,28 @@ tsdf_status tsdf_clean(tsdf_grid *g, uint64_t *n_removed, void *stream);
 tsdf_status tsdf_extract_mesh(tsdf_grid *g, float *triangles, float *colors, uint64_t *vertex_keys, uint64_t cap,
                               uint64_t *n_triangles, void *stream);
 
+/* Marching cubes: the mesh of marchingCubesHost, GPUVoxelGrid.cpp:200-251
+ * (Open3D 0.12 ScalableTSDFVolume::ExtractTriangleMesh over the bricks, then
+ * ComputeVertexNormals), as an indexed mesh.  A cell of 8 voxels is meshed
+ * when no corner has w == 0, by the Lorensen-Cline case tables; one vertex
+ * per lattice edge (global voxel G, axis a), at vl/2 + vl G plus
+ * f0 vl / (f0 + f1) along a (doubles, f = |sdf| at the edge's ends), with
+ * colour (f1 c0 + f0 c1) / (f0 + f1) of c = rgb / 255; triangles face the
+ * free-space side; normals are the normalised sums of the unnormalised face
+ * normals (zero where the sum is zero).  DEVICE outputs: vertices
+ * [vertex_cap][3] double; vertex_colors [vertex_cap][3] double or NULL (a
+ * grid without colour accepts only NULL); vertex_normals [vertex_cap][3]
+ * double or NULL; triangles [triangle_cap][3] int32 vertex indices;
+ * vertex_keys [vertex_cap] or NULL, the edge of each vertex (G biased by
+ * 2^18 per axis in bits 43-61 / 24-42 / 5-23, a in bits 0-4; maps beyond
+ * +-32767 blocks per axis are refused).  *n_vertices and *n_triangles (host)
+ * receive the mesh's sizes; nothing is written unless both fit their
+ * capacities.  Vertex order is not fixed from call to call; the vertex set
+ * and the triangle multiset are.  Synchronous. */
+tsdf_status tsdf_marching_cubes(tsdf_grid *g, double *vertices, double *vertex_colors, double *vertex_normals,
+                                int32_t *triangles, uint64_t *vertex_keys, uint64_t vertex_cap,
+                                uint64_t triangle_cap, uint64_t *n_vertices, uint64_t *n_triangles, void *stream);
+
 /* ---- point maps, normal maps and projective ICP (kokkosICP.cpp) ---------- */
 
 typedef enum {

@@ -35,7 +35,7 @@ EXPORTS = (
     "tsdf_last_error", "tsdf_raycast_footprint", "tsdf_profile", "tsdf_profile_read",
     "tsdf_default_tuning", "tsdf_get_tuning", "tsdf_set_tuning",
     "tsdf_render", "tsdf_render_backward", "tsdf_track_normal_eq", "tsdf_se3_gauss_newton_step",
-    "tsdf_clean", "tsdf_extract_mesh",
+    "tsdf_clean", "tsdf_extract_mesh", "tsdf_marching_cubes",
     "tsdf_icp_default_params", "tsdf_depth_to_points", "tsdf_points_to_normals", "tsdf_icp_associate",
     "tsdf_icp_moments", "tsdf_icp_moments_noalloc", "tsdf_icp", "tsdf_icp_release",
     "tsdf_mat_to_dof", "tsdf_dof_to_mat", "tsdf_reint_diff", "tsdf_reint_select",
@@ -155,6 +155,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tsdf_se3_gauss_newton_step.argtypes = [dp, dp, dp, dp, dp]
     L.tsdf_clean.argtypes = [vp, P(C.c_uint64), vp]
     L.tsdf_extract_mesh.argtypes = [vp, vp, vp, vp, C.c_uint64, P(C.c_uint64), vp]
+    L.tsdf_marching_cubes.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint64, C.c_uint64, P(C.c_uint64), P(C.c_uint64),
+                                      vp]
     L.tsdf_icp_default_params.argtypes = [P(TsdfIcpParams)]
     L.tsdf_icp_default_params.restype = None
     L.tsdf_depth_to_points.argtypes = [vp, P(TsdfView), vp, vp]

@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/tsdf.h"
 #include "tsdf_math.hpp"
@@ -2478,5 +2479,6 @@ tsdf_status tsdf_export_bricks(tsdf_grid* g, int32_t* coords, float* sdf, float*
 
 #include "tsdf_render.hpp"
 #include "tsdf_maintain.hpp"
+#include "tsdf_mc.hpp"
 #include "tsdf_icp.hpp"
 #include "tsdf_sched.hpp"

@@ -8,8 +8,9 @@ import numpy as np
 
 def write_ply(path, mesh: dict) -> None:
     """Vertices (float32 x y z), optional unit normals (nx ny nz), optional
-    colours (uint8 red green blue, from 0..255 floats, rounded and clamped) and
-    triangular faces (uchar count, int32 indices)."""
+    colours (uint8 red green blue: colours in 0..color_scale, 255 unless the
+    mesh says otherwise, scaled to 0..255, rounded and clamped) and triangular
+    faces (uchar count, int32 indices)."""
     v = np.ascontiguousarray(mesh["vertices"], np.float32).reshape(-1, 3)
     f = np.ascontiguousarray(mesh["faces"]).reshape(-1, 3)
     nrm = mesh.get("normals")
@@ -25,7 +26,8 @@ def write_ply(path, mesh: dict) -> None:
         nrm = np.asarray(nrm, np.float32).reshape(-1, 3)
         rec["nx"], rec["ny"], rec["nz"] = nrm[:, 0], nrm[:, 1], nrm[:, 2]
     if col is not None:
-        c = np.clip(np.rint(np.asarray(col, np.float64).reshape(-1, 3)), 0, 255).astype(np.uint8)
+        scale = 255.0 / float(mesh.get("color_scale", 255.0))
+        c = np.clip(np.rint(np.asarray(col, np.float64).reshape(-1, 3) * scale), 0, 255).astype(np.uint8)
         rec["red"], rec["green"], rec["blue"] = c[:, 0], c[:, 1], c[:, 2]
     frec = np.zeros(len(f), dtype=[("n", "u1"), ("i", "<i4", (3,))])
     frec["n"] = 3
@@ -45,8 +47,9 @@ FLIP = np.diag([1.0, -1.0, -1.0, 1.0])  # getflip, matrixutil.cpp:27-35
 
 def save_cpu_mesh(path, mesh: dict, flip: bool = True) -> None:
     """Model::saveCPUMesh, Model.cpp:205-228: a copy of the mesh whose colour
-    channels outside the valid range are set to 0 (the reference tests its
-    [0, 1] Open3D colours; these are 0..255), transformed by getflip
+    channels outside the valid range [0, color_scale] are set to 0 (the
+    reference tests its [0, 1] Open3D colours, the marching-cubes colours;
+    marching-tetrahedra colours are 0..255), transformed by getflip
     (vertices and normals; a rotation, so the winding is kept), written as
     binary PLY with normals and colours."""
     m = dict(mesh)
@@ -59,7 +62,7 @@ def save_cpu_mesh(path, mesh: dict, flip: bool = True) -> None:
     col = m.get("colors")
     if col is not None:
         col = np.asarray(col, np.float64).reshape(-1, 3).copy()
-        col[(col > 255.0) | (col < 0.0)] = 0.0
+        col[(col > float(m.get("color_scale", 255.0))) | (col < 0.0)] = 0.0
     m["vertices"] = v.astype(np.float32)
     m["normals"] = nrm.astype(np.float32) if nrm is not None else None
     m["colors"] = col

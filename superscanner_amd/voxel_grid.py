@@ -93,6 +93,7 @@ class GPUVoxelGrid:
         self.maxVoxelBlocks = bricks
         self.withColor = bool(withColor)
         self.intrinsicSet = False
+        self.mesh = None  # the last marchingCubesHost mesh (GPUVoxelGrid.h:60)
         # grid-owned raycast outputs (rayPcd, rayColorPcd, GPUVoxelGrid.h:67-69)
         self.rayPcd = torch.zeros((resi, resj, 3), dtype=torch.float32, device=self.device)
         self.rayColorPcd = torch.zeros((resi, resj, 3), dtype=torch.float32, device=self.device)
@@ -281,15 +282,27 @@ class GPUVoxelGrid:
         check(self._L.tsdf_clean(self._h, C.byref(n), _stream_ptr(stream)), "tsdf_clean")
         return int(n.value)
 
-    def extract_mesh(self, stream=None) -> dict:
-        """Mesh of the zero level set (stands in for marchingCubesHost,
-        GPUVoxelGrid.cpp:200-251, and the ComputeVertexNormals that follows),
-        welded on the GPU by vertex key: {"triangles": [m, 3, 3] device
-        float32 soup, "triangle_colors": [m, 3, 3] device float32 or None,
-        "vertices": [k, 3] float32, "faces": [m, 3] int64, "normals": [k, 3]
-        float32 unit area-weighted vertex normals (zero where every face of
-        the vertex has zero area), "colors": [k, 3] float32 or None} -- the
-        last four as host numpy arrays."""
+    def extract_mesh(self, stream=None, method: str = "cubes") -> dict:
+        """Mesh of the zero level set.
+
+        method "cubes" (the reference's: marchingCubesHost,
+        GPUVoxelGrid.cpp:200-251, Open3D's ExtractTriangleMesh and
+        ComputeVertexNormals), tsdf_marching_cubes: {"vertices": [k, 3]
+        float64, "faces": [m, 3] int64, "normals": [k, 3] float64 unit
+        area-weighted vertex normals (zero where the sum is zero), "colors":
+        [k, 3] float64 in 0..1 (rgb / 255) or None, "vertex_keys": [k] int64
+        lattice edge of each vertex, "color_scale": 1.0} as host numpy arrays.
+
+        method "tetrahedra": marching tetrahedra (tsdf_extract_mesh), welded on
+        the GPU by vertex key: {"triangles": [m, 3, 3] device float32 soup,
+        "triangle_colors": [m, 3, 3] device float32 or None, "vertices":
+        [k, 3] float32, "faces": [m, 3] int64, "normals": [k, 3] float32,
+        "colors": [k, 3] float32 in 0..255 or None, "color_scale": 255.0} --
+        the last four as host numpy arrays."""
+        if method == "cubes":
+            return self._marching_cubes(stream)
+        if method != "tetrahedra":
+            raise ValueError(f"unknown mesh method {method!r} (cubes or tetrahedra)")
         cap = max(1024, 64 * self.size())
         while True:
             tris = torch.empty((cap, 3, 3), dtype=torch.float32, device=self.device)
@@ -325,7 +338,37 @@ class GPUVoxelGrid:
                 vcol.index_copy_(0, inv, cols.reshape(-1, 3))
         return {"triangles": tris, "triangle_colors": cols, "vertices": verts.cpu().numpy(),
                 "faces": faces.cpu().numpy(), "normals": vn.cpu().numpy(),
-                "colors": vcol.cpu().numpy() if vcol is not None else None}
+                "colors": vcol.cpu().numpy() if vcol is not None else None, "color_scale": 255.0}
+
+    def marchingCubesHost(self, stream=None) -> dict:
+        """GPUVoxelGrid::marchingCubesHost (GPUVoxelGrid.cpp:200-251): the
+        marching-cubes mesh, kept as ``self.mesh`` and returned."""
+        self.mesh = self._marching_cubes(stream)
+        return self.mesh
+
+    def _marching_cubes(self, stream=None) -> dict:
+        L, dev = self._L, self.device
+        n_b = max(1, self.size())
+        vcap, tcap = 64 * n_b, 128 * n_b
+        while True:
+            verts = torch.empty((max(vcap, 1), 3), dtype=torch.float64, device=dev)
+            cols = torch.empty((max(vcap, 1), 3), dtype=torch.float64, device=dev) if self.withColor else None
+            nrm = torch.empty((max(vcap, 1), 3), dtype=torch.float64, device=dev)
+            keys = torch.empty((max(vcap, 1),), dtype=torch.int64, device=dev)
+            faces = torch.empty((max(tcap, 1), 3), dtype=torch.int32, device=dev)
+            nv, nt = C.c_uint64(0), C.c_uint64(0)
+            check(L.tsdf_marching_cubes(self._h, C.c_void_p(verts.data_ptr()),
+                                        C.c_void_p(cols.data_ptr()) if cols is not None else None,
+                                        C.c_void_p(nrm.data_ptr()), C.c_void_p(faces.data_ptr()),
+                                        C.c_void_p(keys.data_ptr()), vcap, tcap, C.byref(nv), C.byref(nt),
+                                        _stream_ptr(stream)), "tsdf_marching_cubes")
+            if nv.value <= vcap and nt.value <= tcap:
+                break
+            vcap, tcap = int(nv.value), int(nt.value)
+        k, m = int(nv.value), int(nt.value)
+        return {"vertices": verts[:k].cpu().numpy(), "faces": faces[:m].long().cpu().numpy(),
+                "normals": nrm[:k].cpu().numpy(), "colors": cols[:k].cpu().numpy() if cols is not None else None,
+                "vertex_keys": keys[:k].cpu().numpy(), "color_scale": 1.0}
 
     def sdf_tensor(self) -> torch.Tensor:
         """The stored sdf values as a device float32 tensor [n_bricks, 512]

@@ -48,7 +48,7 @@ def oracle_mesh_of(g):
 
 
 def mesh_with_keys(g):
-    m = g.extract_mesh()
+    m = g.extract_mesh(method="tetrahedra")
     n = m["triangles"].shape[0]
     keys = torch.empty((max(n, 1), 3), dtype=torch.int64, device=g.device)
     tris = torch.empty((max(n, 1), 3, 3), dtype=torch.float32, device=g.device)
@@ -97,7 +97,7 @@ def test_mesh_colours_equal_oracle(gpu):
     for k in range(6):
         rgb = rng.integers(0, 256, size=(240, 320, 3), dtype=np.uint8)
         g.integrate(dev_u8(rgb), dev_u16(depth[k]), poses[k])
-    got = g.extract_mesh()
+    got = g.extract_mesh(method="tetrahedra")
     ex = {k: v.cpu().numpy() for k, v in g.export_bricks().items()}
     t, c = mesh.extract_mesh(ex["coords"], ex["sdf"], ex["w"], g.vl, ex["rgb"])
     tg, cg = got["triangles"].cpu().numpy(), got["triangle_colors"].cpu().numpy()
@@ -129,7 +129,7 @@ def test_mesh_short_buffer(gpu):
     g, _ = make_pair(cfg["voxel_length"], cfg["truncation"], 120, 160, intr)
     for k in range(2):
         g.integrate(None, dev_u16(depth[k]), poses[k])
-    full = g.extract_mesh()["triangles"].cpu().numpy()
+    full = g.extract_mesh(method="tetrahedra")["triangles"].cpu().numpy()
     cap = 100
     buf = torch.full((cap + 1, 3, 3), 7.0, dtype=torch.float32, device=gpu)
     n = C.c_uint64(0)
@@ -151,7 +151,7 @@ def test_mesh_short_buffer(gpu):
 def test_mesh_of_empty_grid(gpu):
     cfg, intr, _, _ = small_frames(n=1)
     g, _ = make_pair(cfg["voxel_length"], cfg["truncation"], 240, 320, intr)
-    m = g.extract_mesh()
+    m = g.extract_mesh(method="tetrahedra")
     assert m["triangles"].shape[0] == 0 and m["faces"].shape == (0, 3)
     g.close()
 
@@ -238,7 +238,7 @@ def test_clean_everything_then_integrate(gpu):
     n = g.size()
     assert g.cleanMemory() == n == o.clean()
     assert g.size() == 0 and g.cleanMemory() == 0
-    assert g.extract_mesh()["triangles"].shape[0] == 0
+    assert g.extract_mesh(method="tetrahedra")["triangles"].shape[0] == 0
     g.integrate(None, dev_u16(depth[1]), poses[1])
     o.integrate(depth[1], poses[1], 1)
     assert_grids_equal(g, o)
