@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=600)
-    ap.add_argument("--cpu-frames", type=int, default=120,
+    ap.add_argument("--cpu-frames", type=int, default=60,
                     help="frames of the sequence the CPU baseline processes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reintegration", action="store_true")
@@ -132,6 +132,123 @@ class Runner:
             raise RuntimeError("a TSDF call failed: " + L.tsdf_last_error().decode())
 
 
+class HostRunner(Runner):
+    """The same step with the depth frames starting in pinned host memory
+    (the reference's Frame::MovetoGPU before each integrate, Frame.cpp:44-60):
+    every integrate / deintegrate uploads its frame inside the timed region.
+    overlap=False: upload, then integrate, in stream order.  overlap=True:
+    a second stream uploads up to `ring` - 1 frames ahead into a ring of
+    device buffers, events ordering each upload after the integrate that
+    last read its buffer.  d2h=True also copies every raycast's points and
+    depth back to pinned host memory after the raycast."""
+
+    def __init__(self, grid, depth_host, poses, frames, ring=4):
+        import torch
+        super().__init__(grid, torch.empty(1, dtype=torch.int16, device=grid.device), poses, frames)
+        H, W = grid.resi, grid.resj
+        self.host = depth_host  # [frames, H, W] int16, pinned
+        self.ring = ring
+        self.slots = [torch.empty((H, W), dtype=torch.int16, device=grid.device) for _ in range(ring)]
+        self.pts_h = torch.empty((H, W, 3), dtype=torch.float32, pin_memory=True)
+        self.dep_h = torch.empty((H, W), dtype=torch.float32, pin_memory=True)
+        self.copy = torch.cuda.Stream(device=grid.device)
+        self.ready = [torch.cuda.Event() for _ in range(ring)]
+        self.free = [torch.cuda.Event() for _ in range(ring)]
+
+    def step_host(self, overlap, d2h=False):
+        import torch
+        L, h, s = self.L, self.h, self.stream
+        comp = torch.cuda.current_stream()
+        pts, dep = C.c_void_p(self.pts.data_ptr()), C.c_void_p(self.dep.data_ptr())
+        uploads = [k for op, k in self.ops if op != "r"]
+        R, D = self.ring, self.ring - 1
+
+        def upload(n):
+            slot = n % R
+            with torch.cuda.stream(self.copy):
+                self.copy.wait_event(self.free[slot])
+                self.slots[slot].copy_(self.host[uploads[n]], non_blocking=True)
+                self.ready[slot].record(self.copy)
+
+        rc = L.tsdf_reset(h, s)
+        if overlap:
+            for slot in range(R):
+                self.free[slot].record(comp)
+            for n in range(min(D, len(uploads))):
+                upload(n)
+        n = 0
+        for op, k in self.ops:
+            if op == "r":
+                rc |= L.tsdf_raycast(h, self.Ep[k], pts, None, dep, None, s)
+                if d2h:
+                    self.pts_h.copy_(self.pts, non_blocking=True)
+                    self.dep_h.copy_(self.dep, non_blocking=True)
+                continue
+            slot = n % R
+            if overlap:
+                comp.wait_event(self.ready[slot])
+            else:
+                self.slots[slot].copy_(self.host[k], non_blocking=True)
+            rc |= L.tsdf_integrate(h, C.c_void_p(self.slots[slot].data_ptr()), self.pitch, None, self.Ep[k],
+                                   1 if op == "i" else -1, s)
+            if overlap:
+                self.free[slot].record(comp)
+                if n + D < len(uploads):
+                    upload(n + D)
+            n += 1
+        if rc:
+            raise RuntimeError("a TSDF call failed: " + L.tsdf_last_error().decode())
+
+
+def host_inclusive(grid, depth, poses, frames, ms_resident, steps=2):
+    """The headline with the frames' host -> device copies inside the timed
+    region (and, in one variant, the raycast outputs copied back)."""
+    import torch
+    host = torch.from_numpy(depth.view(np.int16)).pin_memory()
+    run = HostRunner(grid, host, poses, frames)
+    px = grid.resi * grid.resj
+    out = {"resident_ms_per_step": round(ms_resident, 3)}
+    for name, overlap, d2h in (("h2d_serialized", False, False), ("h2d_overlapped", True, False),
+                               ("h2d_d2h_serialized", False, True)):
+        run.step_host(overlap, d2h)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run.step_host(overlap, d2h)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        out[name] = {"ms_per_step": round(ms, 3), "Mpx_per_s": round(frames * px / (ms * 1e-3) / 1e6, 2)}
+    out["note"] = ("h2d_*: every integrate / deintegrate uploads its uint16 frame from pinned host memory in the "
+                   "timed region (serialized: in stream order; overlapped: a copy stream runs 3 frames ahead); "
+                   "h2d_d2h_serialized also copies each raycast's points + depth (16 B/px) back to pinned host "
+                   "memory; wall clock over %d steps" % steps)
+    return out
+
+
+def device_copy(dev, nbytes=1 << 30, reps=10):
+    """Measured device-to-device copy rate (bytes read + written per second):
+    hipMemcpy D2D through torch and a plain elementwise copy kernel."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    out = {"bytes": nbytes}
+    for name, fn in (("memcpy_d2d", lambda: b.copy_(a)), ("copy_kernel", lambda: torch.mul(a, 1.0, out=b))):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        out[name + "_GBs"] = round(2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    out["device_copy_GBs"] = max(out["memcpy_d2d_GBs"], out["copy_kernel_GBs"])
+    out["peak_GBs"] = HBM_PEAK_GBS
+    out["copy_frac_of_peak"] = round(out["device_copy_GBs"] / HBM_PEAK_GBS, 3)
+    del a, b
+    return out
+
+
 def autotune_plain(cfg, depth_dev, poses, frames):
     """Launch shapes of the plain integrate and raycast, picked by the
     autotuner on a scratch grid holding the first part of the sequence."""
@@ -171,26 +288,55 @@ def footprint_pass(runner, n_samples):
     return float(np.mean(u_vox)), float(np.mean(u_blk))
 
 
-def cpu_baseline(cfg, poses, depth, n):
+def host_cpu():
+    """nproc, the cores this process may run on, and the CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_baseline(cfg, poses, depth, n, runs=5):
+    """The oracle over the first n frames of the sequence on the host cores
+    this process is allotted (OMP_NUM_THREADS where set, else its CPU
+    affinity): one warm-up pass, then the median of `runs` passes."""
+    import statistics
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = int(env) if env and env.isdigit() and int(env) > 0 else host["affinity"]
     oracle.set_threads(threads)
-    o = oracle.OracleGrid(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], max_bricks=1 << 22,
-                          max_weight=cfg["max_weight"])
-    o.set_intrinsics(*cfg["intr"])
     ops = schedule(n)
-    t0 = time.perf_counter()
-    for op, k in ops:
-        if op == "r":
-            o.raycast(poses[k])
-        else:
-            o.integrate(depth[k], poses[k], 1 if op == "i" else -1)
-    t = time.perf_counter() - t0
-    o.close()
+
+    def one():
+        o = oracle.OracleGrid(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], max_bricks=1 << 22,
+                              max_weight=cfg["max_weight"])
+        o.set_intrinsics(*cfg["intr"])
+        t0 = time.perf_counter()
+        for op, k in ops:
+            if op == "r":
+                o.raycast(poses[k])
+            else:
+                o.integrate(depth[k], poses[k], 1 if op == "i" else -1)
+        t = time.perf_counter() - t0
+        o.close()
+        return t
+
+    one()
+    times = [one() for _ in range(runs)]
+    t = statistics.median(times)
     return {"value": round(n * cfg["H"] * cfg["W"] / t / 1e6, 3), "unit": "Mpx/s", "cores": threads,
             "kind": "port",
             "sample": f"first {n} of the 600 poses of the same sequence (integrate, deintegrate every 10th, "
-                      f"raycast each pose), {t:.1f} s wall on {threads} OpenMP threads"}
+                      f"raycast each pose): median of {runs} passes after one warm-up, {t:.2f} s per pass, on "
+                      f"{threads} OpenMP threads",
+            "runs_s": [round(x, 3) for x in times], "host": host,
+            "threads_from": "OMP_NUM_THREADS" if env else "sched_getaffinity"}
 
 
 def reintegration(cfg, prims, poses64, keyframes, dev):
@@ -687,6 +833,11 @@ def main():
     kt = grid.profile_read(reset=True)
     grid.profile(False)
 
+    hbm = device_copy(dev)
+    print(f"[bench] device copy {hbm['device_copy_GBs']} GB/s", file=sys.stderr, flush=True)
+    host_incl = host_inclusive(grid, depth, poses, frames, ms)
+    print(f"[bench] host-inclusive steps done", file=sys.stderr, flush=True)
+
     u_vox, u_blk = footprint_pass(run, args.footprint_samples)
     n_int = sum(1 for o, _ in run.ops if o in "id")
     n_ray = sum(1 for o, _ in run.ops if o == "r")
@@ -720,7 +871,9 @@ def main():
             traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
     roof = {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kernels[dom]["frac"], "traffic": traffic,
-            "bytes_per_launch": kernels[dom]["bytes_per_launch"], "avg_us": kernels[dom]["avg_us"]}
+            "bytes_per_launch": kernels[dom]["bytes_per_launch"], "avg_us": kernels[dom]["avg_us"],
+            "measured_copy_GBs": hbm["device_copy_GBs"],
+            "frac_of_measured_copy": round(kernels[dom]["achieved_GBs"] / hbm["device_copy_GBs"], 4)}
 
     value = frames * px / (ms * 1e-3) / 1e6
     out = {
@@ -739,6 +892,8 @@ def main():
         "ms_per_step_profiled": round(ms_prof, 3),
         "raycast_footprint": {"u_vox": round(u_vox), "u_blk": round(u_blk), "samples": args.footprint_samples},
         "tuning": tuning,
+        "hbm": hbm,
+        "host_inclusive": host_incl,
     }
     if not args.no_maintenance:
         out["maintenance"] = maintenance_leg(grid)
