@@ -155,6 +155,36 @@ def test_deferred_rays_room_and_colour(gpu, budget):
         assert after["ray_probes"] - before["ray_probes"] == r["counts"]["probes"]
 
 
+def test_raycast_after_deintegration_and_batch(gpu):
+    """Room frames with deintegrations, then a batched reintegration that
+    deintegrates frames at poses they were never integrated at (weights go
+    to 0 and below): the raycast equals the oracle's, counts included, with
+    and without deferred rays.  Zero crossings between two zero samples give
+    NaN points on both sides."""
+    cfg, intr, poses, depth = room(14, H=240, W=424)
+    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], 240, 424, intr, max_weight=128.0)
+    for k in range(12):
+        g.integrate(None, dev_u16(depth[k]), poses[k])
+        o.integrate(depth[k], poses[k], 1)
+    for k in (3, 7):
+        g.deIntegrate(None, dev_u16(depth[k]), poses[k])
+        o.integrate(depth[k], poses[k], -1)
+    for budget in (0, 12):
+        g.set_tuning(raycast_budget=budget)
+        for k in (2, 11):
+            before = g.stats()
+            r = assert_raycast_equal(g, o, poses[k])
+            after = g.stats()
+            assert after["ray_samples"] - before["ray_samples"] == r["counts"]["samples"]
+            assert after["ray_probes"] - before["ray_probes"] == r["counts"]["probes"]
+    g.reintegrate_batch([dev_u16(depth[k]) for k in (12, 13)], [poses[0], poses[1]], [poses[12], poses[13]])
+    for old_k, new_k in ((0, 12), (1, 13)):
+        o.integrate(depth[new_k], poses[old_k], -1)
+        o.integrate(depth[new_k], poses[new_k], 1)
+    assert_grids_equal(g, o)
+    assert_raycast_equal(g, o, poses[12])
+
+
 def test_deintegrate_sequence(gpu):
     cfg, intr, poses, depth = plain_small(n=10, H=240, W=320)
     g, o = make_pair(cfg["voxel_length"], cfg["truncation"], 240, 320, intr)
