@@ -483,6 +483,7 @@ constexpr int kRayUbMissingOut = 2;  // TSDF_RAY_UB_MISSING_OUT
 #endif
 constexpr int kSkipLook = TSDF_SKIP_LOOK;    // empty-block probes in flight per lane
 constexpr int kInnerLook = TSDF_INNER_LOOK;  // voxel reads in flight per lane
+static_assert(kInnerLook == 4, "the walk's block search is written for windows of 4 positions");
 
 // Lower bound of the Chebyshev distance, in blocks, from block (x, y, z) to
 // every inserted block: outside the box of set superblocks the distance to
@@ -887,42 +888,60 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
         vin3(rv, bx, by, bz, rp, vx, vy, vz);
         while (true) {
             ++rounds;
-            // voxel index of positions rv, rv + vl dir, ...: -2 where the
-            // loop test fails (range or block left), -1 where the index is
-            // outside [0, 8) (reference UB), else x + 8y + 64z
+            // positions r[0] = rv, r[k] = r[k - 1] + vl dir, and their
+            // lengths, by the reference's additions
+            f3 r[kInnerLook + 1];
+            float ll[kInnerLook + 1];
+            r[0] = rv;
+            ll[0] = l;
+#pragma unroll
+            for (int k = 1; k <= kInnerLook; ++k) {
+                r[k] = add3(r[k - 1], step);
+                ll[k] = ll[k - 1] + vl;
+            }
+            // kl: the first of positions 1..kInnerLook outside block b
+            // (kInnerLook + 1: none), and its block (e*).  Along each axis
+            // the positions are monotone (float sums of a constant step) and
+            // getVBIndeces is monotone (ray_index), and r[0] lies in b: once
+            // a position is outside b every later one is, so the last
+            // position decides and a binary search finds kl.
+            int q4x, q4y, q4z;
+            vb3(r[kInnerLook], rp, q4x, q4y, q4z);
+            const bool in4 = q4x == bx && q4y == by && q4z == bz;
+            int kl = kInnerLook + 1, ex = q4x, ey = q4y, ez = q4z;
+            if (!in4) {
+                int q2x, q2y, q2z;
+                vb3(r[2], rp, q2x, q2y, q2z);
+                const bool in2 = q2x == bx && q2y == by && q2z == bz;
+                int qcx, qcy, qcz;
+                vb3(sel3(in2, r[3], r[1]), rp, qcx, qcy, qcz);
+                const bool inc = qcx == bx && qcy == by && qcz == bz;
+                kl = in2 ? (inc ? 4 : 3) : (inc ? 2 : 1);
+                const bool use2 = !in2 && inc;  // leaves at 2
+                const bool usec = !inc;         // leaves at 1 or 3
+                ex = usec ? qcx : (use2 ? q2x : q4x);
+                ey = usec ? qcy : (use2 ? q2y : q4y);
+                ez = usec ? qcz : (use2 ? q2z : q4z);
+            }
+            // the loop test of position k: in the block and l < d_max (the
+            // lengths increase, so the first failure ends the window)
+            bool e_found = false;  // the first position outside b is reached
+#pragma unroll
+            for (int k = 1; k <= kInnerLook; ++k)
+                e_found = e_found || (kl == k && ll[k - 1] < dmax);
+            // voxel index of positions 0..kInnerLook-1: -2 where the loop
+            // test fails, -1 where the index is outside [0, 8) (reference
+            // UB), else x + 8y + 64z
             int vk[kInnerLook];
             float sk[kInnerLook], wk[kInnerLook];
-            // the first position of this window or the next one outside the
-            // block: its lookup is loaded with the voxels, for the outer loop
-            bool e_found = false;
-            int ex = 0, ey = 0, ez = 0;
-            {
-                f3 r = rv;
-                float ll = l;
-                bool same = l < dmax;
-                int wx = vx, wy = vy, wz = vz;
 #pragma unroll
-                for (int k = 0; k <= kInnerLook; ++k) {
-                    if (k > 0) {
-                        r = add3(r, step);
-                        ll += vl;
-                        int qx, qy, qz;
-                        vb3(r, rp, qx, qy, qz);
-                        const bool inb = qx == bx && qy == by && qz == bz;
-                        const bool leave = same && !inb;
-                        ex = leave ? qx : ex;
-                        ey = leave ? qy : ey;
-                        ez = leave ? qz : ez;
-                        e_found = e_found || leave;
-                        same = same && inb && ll < dmax;
-                        if (k < kInnerLook)
-                            vin3(r, bx, by, bz, rp, wx, wy, wz);
-                    }
-                    if (k < kInnerLook) {
-                        vk[k] = !same ? -2 : (vin_ok(wx, wy, wz) ? wx + kVPE * wy + kVPE * kVPE * wz : -1);
-                        ld_pair(brick, vk[k] >= 0 ? vk[k] : 0, sk[k], wk[k]);
-                    }
-                }
+            for (int k = 0; k < kInnerLook; ++k) {
+                int wx = vx, wy = vy, wz = vz;
+                if (k > 0)
+                    vin3(r[k], bx, by, bz, rp, wx, wy, wz);
+                const bool same = k < kl && ll[k] < dmax;
+                vk[k] = !same ? -2 : (vin_ok(wx, wy, wz) ? wx + kVPE * wy + kVPE * kVPE * wz : -1);
+                ld_pair(brick, vk[k] >= 0 ? vk[k] : 0, sk[k], wk[k]);
             }
             uint32_t e_key = 0, e_bk = 0;
             unsigned long long e_slot = 0;
@@ -933,6 +952,7 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
             }
             bool stop = false;
             int ending = 0;  // 1 left the block / range, 2 UB index, 3 Fn, 4 crossing
+            int nst = 0;     // positions taken (the march goes on from r[nst])
             float xs = 0.f, xw = 0.f;
             int xv = 0;
 #pragma unroll
@@ -952,14 +972,16 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
                 xw = proc ? wk[k] : xw;
                 xv = proc ? vk[k] : xv;
                 Lp = stp ? wpos : Lp;
-                rv = sel3(stp, add3(rv, step), rv);
-                l = stp ? l + vl : l;
+                nst = stp ? k + 1 : nst;
                 stop = stop || !stp;
             }
-            if (!stop) {  // walked past the window: position, block and index of rv
-                int nx, ny, nz;
-                vb3(rv, rp, nx, ny, nz);
-                if (nx != bx || ny != by || nz != bz || !(l < dmax)) {
+#pragma unroll
+            for (int k = 1; k <= kInnerLook; ++k) {
+                rv = sel3(nst == k, r[k], rv);
+                l = nst == k ? ll[k] : l;
+            }
+            if (!stop) {  // walked past the window: rv = r[kInnerLook]
+                if (!in4 || !(l < dmax)) {
                     // the inner loop test fails; rv is the window's next
                     // position, whose lookup was loaded if it left the block
                     pre_valid = e_found;
