@@ -545,7 +545,9 @@ def fp16_scale_leg(dev, frames):
 
 
 CAL_BYTES = 256 << 20   # calibration copy: 256 MiB read + 256 MiB written
-KERNEL_NAMES = {"collect": "k_collect(", "integrate": "k_integrate<", "raycast": "k_raycast<"}
+KERNEL_NAMES = {"collect": "k_collect(", "integrate": "k_integrate<", "ray_bounds": "k_ray_bounds<",
+                "ray_march": "k_raycast<", "ray_tail": "k_raycast_tail<"}
+RAYCAST_PARTS = ("ray_bounds", "ray_march", "ray_tail")  # the kernels of one tsdf_raycast
 
 
 def maintenance_leg(grid):
@@ -764,6 +766,13 @@ def summarize_pmc(fetch_csv, write_csv):
             out[k] = {"fetch_bytes_per_launch": int(fb), "write_bytes_per_launch": int(wb),
                       "hbm_bytes_per_launch": int(fb + wb)}
             lines.append(f"{k:10s} fetch {fb / 1e6:9.2f} MB  write {wb / 1e6:9.2f} MB  total {(fb + wb) / 1e6:9.2f} MB")
+    parts = [out[k] for k in RAYCAST_PARTS if k in out]
+    if parts:  # one tsdf_raycast: the bounds, the per-lane march and the deferred-ray march
+        out["raycast"] = {f: sum(p[f] for p in parts) for f in parts[0]}
+        r = out["raycast"]
+        lines.append(f"{'raycast':10s} fetch {r['fetch_bytes_per_launch'] / 1e6:9.2f} MB  write "
+                     f"{r['write_bytes_per_launch'] / 1e6:9.2f} MB  total {r['hbm_bytes_per_launch'] / 1e6:9.2f} MB"
+                     "  (the three kernels of one call)")
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     with open(os.path.join(ROOT, "profiles", "pmc_traffic.json"), "w") as f:
         json.dump(out, f, indent=1)

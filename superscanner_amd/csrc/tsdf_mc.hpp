@@ -488,21 +488,37 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(GridView g, unsigned int n
     }
 }
 
-// Pass 3: a vertex id for every occupied slot of the weld table.
+// Pass 3: a vertex id for every occupied slot of the weld table.  A
+// workgroup numbers a run of kMcIdsPer slots with one atomic on the
+// counter (a counter taken by every wavefront would queue the atomics of
+// the whole chip on one address).
+constexpr int kMcIdsPer = 16;  // slots per thread and run
 __global__ __launch_bounds__(kThreads) void k_mc_ids(const unsigned long long* hkeys, uint32_t* hids, size_t cap,
                                                      unsigned long long* counter)
 {
-    const size_t stride = (size_t)gridDim.x * kThreads;
-    for (size_t base = (size_t)blockIdx.x * kThreads; base < cap; base += stride) {
-        const size_t i = base + threadIdx.x;
-        const bool occ = i < cap && hkeys[i] != 0ull;
-        const unsigned long long m = __ballot(occ);
-        unsigned long long first = 0;
-        if (__lane_id() == 0 && m)
-            first = atomicAdd(counter, (unsigned long long)__popcll(m));
-        first = __shfl(first, 0);
-        if (occ)
-            hids[i] = (uint32_t)(first + __popcll(m & ((1ull << __lane_id()) - 1ull)));
+    __shared__ unsigned int red[kThreads / 64];
+    __shared__ unsigned long long wg_base;
+    const size_t run = (size_t)kThreads * kMcIdsPer;
+    for (size_t base = (size_t)blockIdx.x * run; base < cap; base += (size_t)gridDim.x * run) {
+        unsigned int mine = 0;  // occupied slots of this thread: base + threadIdx.x + kThreads k
+        for (int k = 0; k < kMcIdsPer; ++k) {
+            const size_t i = base + threadIdx.x + (size_t)kThreads * k;
+            mine |= (i < cap && hkeys[i] != 0ull ? 1u : 0u) << k;
+        }
+        // ids in thread order within the run (vertex order is free)
+        unsigned int total;
+        const unsigned int cnt = (unsigned int)__popc(mine);
+        const unsigned int before = mc_block_scan(cnt, red, &total) - cnt;
+        if (threadIdx.x == 0)
+            wg_base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+        __syncthreads();
+        unsigned long long next = wg_base + before;
+        for (int k = 0; k < kMcIdsPer; ++k) {
+            const size_t i = base + threadIdx.x + (size_t)kThreads * k;
+            if ((mine >> k) & 1u)
+                hids[i] = (uint32_t)(next++);
+        }
+        __syncthreads();
     }
 }
 
@@ -594,10 +610,12 @@ __global__ __launch_bounds__(kThreads) void k_mc_face_normals(const double* vert
             b[k] = verts[3 * (size_t)i2 + k] - verts[3 * (size_t)i0 + k];
         }
         const double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        // the hardware fp64 add (device memory of this process; the order of
+        // the additions is free anyway)
         for (int k = 0; k < 3; ++k) {
-            atomicAdd(&vn[3 * (size_t)i0 + k], n[k]);
-            atomicAdd(&vn[3 * (size_t)i1 + k], n[k]);
-            atomicAdd(&vn[3 * (size_t)i2 + k], n[k]);
+            unsafeAtomicAdd(&vn[3 * (size_t)i0 + k], n[k]);
+            unsafeAtomicAdd(&vn[3 * (size_t)i1 + k], n[k]);
+            unsafeAtomicAdd(&vn[3 * (size_t)i2 + k], n[k]);
         }
     }
 }
