@@ -553,10 +553,61 @@ struct LaneVoxels {
 //     +1: s = (s w + sdf) / (w + 1), w = w + 1 (capped at max_weight, project
 //         semantics: the reference never caps, GPUVoxelGrid.h:14,62)
 //     -1: w != 1: s = (s w - sdf) / (w - 1), w = w - 1; else s = w = 0
-// Returns a bit per voxel that was updated.
-template <bool kColor, int kSign, int kHalf = 4>
-__device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc& op, float vmx, float vmy,
-                                                 const float (&vmz)[kVPE], LaneVoxels<kColor>& v)
+
+// The update of voxel z of a lane (in the band of the op: pix >= 0 and
+// |sdf| < trunc) from its register values.
+template <bool kColor, int kSign>
+__device__ __forceinline__ void update_voxel(const GridView& g, const OpDesc& op, int z, float sdf, int pix,
+                                             LaneVoxels<kColor>& v)
+{
+    const bool use_rgb = kColor && op.rgb != nullptr;
+    const float s = v.s[z], w = v.w[z];
+    float px[3] = {0.f, 0.f, 0.f};
+    if (use_rgb) {
+        const uint8_t* q = op.rgb + (size_t)pix * 3;
+        px[0] = (float)q[0];
+        px[1] = (float)q[1];
+        px[2] = (float)q[2];
+    }
+    if (kSign == 0) {
+        // sign known only at run time (uniform per op): a - b is
+        // computed as a + (-b), which IEEE arithmetic makes identical
+        const bool pos = op.sign > 0;
+        const float sg = pos ? 1.0f : -1.0f;
+        const float d = w + sg;
+        const bool zero = !pos && w == 1.0f;
+        const bool cap = pos && g.wmax > 0.0f && w >= g.wmax;
+        const float q = (s * w + sg * sdf) / d;
+        v.s[z] = zero ? 0.0f : q;
+        if (use_rgb)
+            for (int ch = 0; ch < 3; ++ch)
+                v.c[z][ch] = zero ? 0.0f : (v.c[z][ch] * w + sg * px[ch]) / d;
+        v.w[z] = zero ? 0.0f : (cap ? g.wmax : d);
+    } else if (kSign > 0) {
+        const float d = w + 1.0f;
+        v.s[z] = (s * w + sdf) / d;
+        if (use_rgb)
+            for (int ch = 0; ch < 3; ++ch)
+                v.c[z][ch] = (v.c[z][ch] * w + px[ch]) / d;
+        v.w[z] = (g.wmax > 0.0f && w >= g.wmax) ? g.wmax : d;
+    } else {
+        const float d = w - 1.0f;
+        const bool one = w == 1.0f;
+        v.s[z] = one ? 0.0f : (s * w - sdf) / d;
+        if (use_rgb)
+            for (int ch = 0; ch < 3; ++ch)
+                v.c[z][ch] = one ? 0.0f : (v.c[z][ch] * w - px[ch]) / d;
+        v.w[z] = one ? 0.0f : d;
+    }
+}
+
+// The sdf and pixel of voxels h .. h + kN - 1 of a lane: the projections
+// first, then the point-map gathers together, then the sdf.  Returns the
+// band bits (pix >= 0 and |sdf| < trunc).
+template <int kN>
+__device__ __forceinline__ unsigned int op_band(const GridView& g, const OpDesc& op, float vmx, float vmy,
+                                                const float (&vmz)[kVPE], int h, float (&sdf)[kVPE],
+                                                int (&pix)[kVPE])
 {
     const float* Ei = op.pose.Ei;
     // transformPoint with the z-independent part hoisted; the association
@@ -565,69 +616,44 @@ __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc
     const float a1 = vmx * Ei[4] + vmy * Ei[5];
     const float a2 = vmx * Ei[8] + vmy * Ei[9];
     const float ez0 = op.pose.Ez[0], ez1 = op.pose.Ez[1], ez2 = op.pose.Ez[2];
-    const bool use_rgb = kColor && op.rgb != nullptr;
-    unsigned int upd = 0;
-    // two halves of kHalf voxels: the gathers of a half are issued together
 #pragma unroll
-    for (int h = 0; h < kVPE; h += kHalf) {
-    int pix[kVPE];
-#pragma unroll
-    for (int z = h; z < h + kHalf; ++z)
+    for (int z = h; z < h + kN; ++z)
         pix[z] = project_voxel(g, a0 + vmz[z] * Ei[2] + Ei[3], a1 + vmz[z] * Ei[6] + Ei[7],
                                a2 + vmz[z] * Ei[10] + Ei[11]);
     float pt[kVPE][3];
 #pragma unroll
-    for (int z = h; z < h + kHalf; ++z) {
+    for (int z = h; z < h + kN; ++z) {
         const float* q = (const float*)(op.ptmap + (pix[z] >= 0 ? pix[z] : 0));
         pt[z][0] = q[0];
         pt[z][1] = q[1];
         pt[z][2] = q[2];
     }
+    unsigned int band = 0;
 #pragma unroll
-    for (int z = h; z < h + kHalf; ++z) {
-        const float sdf = (pt[z][0] - vmx) * ez0 + (pt[z][1] - vmy) * ez1 + (pt[z][2] - vmz[z]) * ez2;
-        if (pix[z] >= 0 && fabsf(sdf) < g.trunc) {
-            upd |= 1u << z;
-            const float s = v.s[z], w = v.w[z];
-            float px[3] = {0.f, 0.f, 0.f};
-            if (use_rgb) {
-                const uint8_t* q = op.rgb + (size_t)pix[z] * 3;
-                px[0] = (float)q[0];
-                px[1] = (float)q[1];
-                px[2] = (float)q[2];
-            }
-            if (kSign == 0) {
-                // sign known only at run time (uniform per op): a - b is
-                // computed as a + (-b), which IEEE arithmetic makes identical
-                const bool pos = op.sign > 0;
-                const float sg = pos ? 1.0f : -1.0f;
-                const float d = w + sg;
-                const bool zero = !pos && w == 1.0f;
-                const bool cap = pos && g.wmax > 0.0f && w >= g.wmax;
-                const float q = (s * w + sg * sdf) / d;
-                v.s[z] = zero ? 0.0f : q;
-                if (use_rgb)
-                    for (int ch = 0; ch < 3; ++ch)
-                        v.c[z][ch] = zero ? 0.0f : (v.c[z][ch] * w + sg * px[ch]) / d;
-                v.w[z] = zero ? 0.0f : (cap ? g.wmax : d);
-            } else if (kSign > 0) {
-                const float d = w + 1.0f;
-                v.s[z] = (s * w + sdf) / d;
-                if (use_rgb)
-                    for (int ch = 0; ch < 3; ++ch)
-                        v.c[z][ch] = (v.c[z][ch] * w + px[ch]) / d;
-                v.w[z] = (g.wmax > 0.0f && w >= g.wmax) ? g.wmax : d;
-            } else {
-                const float d = w - 1.0f;
-                const bool one = w == 1.0f;
-                v.s[z] = one ? 0.0f : (s * w - sdf) / d;
-                if (use_rgb)
-                    for (int ch = 0; ch < 3; ++ch)
-                        v.c[z][ch] = one ? 0.0f : (v.c[z][ch] * w - px[ch]) / d;
-                v.w[z] = one ? 0.0f : d;
-            }
-        }
+    for (int z = h; z < h + kN; ++z) {
+        sdf[z] = (pt[z][0] - vmx) * ez0 + (pt[z][1] - vmy) * ez1 + (pt[z][2] - vmz[z]) * ez2;
+        band |= (pix[z] >= 0 && fabsf(sdf[z]) < g.trunc) ? 1u << z : 0u;
     }
+    return band;
+}
+
+// The op applied to a lane's eight voxels held in registers, in two halves
+// of kHalf voxels.  Returns a bit per voxel that was updated.
+template <bool kColor, int kSign, int kHalf = 4>
+__device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc& op, float vmx, float vmy,
+                                                 const float (&vmz)[kVPE], LaneVoxels<kColor>& v)
+{
+    unsigned int upd = 0;
+#pragma unroll
+    for (int h = 0; h < kVPE; h += kHalf) {
+        float sdf[kVPE];
+        int pix[kVPE];
+        const unsigned int band = op_band<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix);
+#pragma unroll
+        for (int z = h; z < h + kHalf; ++z)
+            if ((band >> z) & 1u)
+                update_voxel<kColor, kSign>(g, op, z, sdf[z], pix[z], v);
+        upd |= band;
     }
     return upd;
 }
@@ -662,6 +688,7 @@ __device__ __forceinline__ void load_lane(const GridView& g, int id, int lane, b
 #ifndef TSDF_STORE_LINES
 #define TSDF_STORE_LINES 1
 #endif
+
 __device__ __forceinline__ bool line_dirty(bool mine, int lane, int lanes_per_line)
 {
     const unsigned long long b = __ballot(mine);
@@ -1029,7 +1056,10 @@ __device__ __forceinline__ void write_ray(const RayResult& r, size_t o, const Po
 // k + groups, ... (a shared counter would put every group's atomic on one
 // address).  The first thread clears the other parity's count for the next
 // raycast.
-constexpr int kRayGroup = 16;
+#ifndef TSDF_RAY_GROUP
+#define TSDF_RAY_GROUP 16
+#endif
+constexpr int kRayGroup = TSDF_RAY_GROUP;
 
 template <typename T, bool kColor>
 __global__ __launch_bounds__(kThreads) void k_raycast_tail(GridView g, Pose pose, RayParams rp, float* points,
