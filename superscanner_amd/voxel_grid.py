@@ -376,7 +376,15 @@ class GPUVoxelGrid:
         differentiable render (superscanner_amd.diff_render)."""
         return self.export_bricks()["sdf"]
 
-    def export_bricks(self, stream=None) -> dict:
+    def export_bricks(self, stream=None, layout: str = "storage") -> dict:
+        """The map as device tensors: coords [n, 3] int32 block indices, sdf and
+        w [n, 512], rgb [n, 512, 3] (colour grids).  layout "storage": voxel
+        x + 8y + 64z (GPUVoxelGrid.h:28-31); "open3d": voxel 64x + 8y + z, the
+        index Open3D's UniformTSDFVolume::IndexOf gives the same voxel in the
+        volume marchingCubesHost builds (GPUVoxelGrid.cpp:226-241), i.e. the
+        wire format of a unit."""
+        if layout not in ("storage", "open3d"):
+            raise ValueError(f"unknown layout {layout!r} (storage or open3d)")
         n = self.size()
         dev = self.device
         coords = torch.empty((max(n, 1), 3), dtype=torch.int32, device=dev)
@@ -392,7 +400,23 @@ class GPUVoxelGrid:
         out = {"coords": coords[:m], "sdf": sdf[:m], "w": w[:m]}
         if rgb is not None:
             out["rgb"] = rgb[:m]
+        if layout == "open3d":
+            out = {k: (v if k == "coords" else storage_to_open3d(v)) for k, v in out.items()}
         return out
+
+
+def storage_to_open3d(a):
+    """Per-brick voxel arrays [n, 512, ...] from the storage index
+    x + 8y + 64z to Open3D's 64x + 8y + z (a torch tensor or numpy array;
+    trailing dimensions, e.g. the three colour channels, are kept)."""
+    n = a.shape[0]
+    rest = tuple(a.shape[2:])
+    v = a.reshape((n, 8, 8, 8) + rest)  # [n, z, y, x, ...]
+    if isinstance(v, torch.Tensor):
+        v = v.transpose(1, 3).contiguous()
+    else:
+        v = np.ascontiguousarray(np.swapaxes(v, 1, 3))
+    return v.reshape((n, 512) + rest)
 
 
 def pose_prepare(E):
