@@ -1056,6 +1056,9 @@ __device__ __forceinline__ void write_ray(const RayResult& r, size_t o, const Po
 // k + groups, ... (a shared counter would put every group's atomic on one
 // address).  The first thread clears the other parity's count for the next
 // raycast.
+#ifndef TSDF_TAIL_WG_PER_CU
+#define TSDF_TAIL_WG_PER_CU 8  // workgroups of the deferred-ray march per CU
+#endif
 #ifndef TSDF_RAY_GROUP
 #define TSDF_RAY_GROUP 16
 #endif
@@ -1780,7 +1783,7 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
     if (rp.budget > 0) {
         // the deferred rays, kRayGroup lanes each (the queue length is read
         // on the device; groups past its end leave at once)
-        const dim3 tgrid(g->n_cu * 4);
+        const dim3 tgrid(g->n_cu * TSDF_TAIL_WG_PER_CU);
         if (g->p.storage == TSDF_STORAGE_F16) {
             if (color)
                 hipLaunchKernelGGL((k_raycast_tail<__half, true>), tgrid, dim3(kThreads), 0, s, view(g), P, rp,
