@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="turn two rocprofv3 counter_collection.csv files of --pmc-workload runs into "
                          "profiles/pmc_traffic.json and profiles/pmc_traffic_room_large.txt")
+    ap.add_argument("--summarize-sq", nargs="+", metavar="DB",
+                    help="print per-kernel medians of the counters in rocprofv3 --pmc result databases "
+                         "(run_results.db of --pmc-workload runs) as JSON")
     return ap.parse_args()
 
 
@@ -781,8 +784,36 @@ def summarize_pmc(fetch_csv, write_csv):
     print("\n".join(lines))
 
 
+def summarize_sq(dbs):
+    """Per-kernel medians (over dispatches) of every counter in rocprofv3's
+    SQLite results, with the median dispatch duration."""
+    import collections
+    import sqlite3
+    import statistics
+    out = {}
+    for db in dbs:
+        c = sqlite3.connect(db)
+        rows = c.execute("select kernel_name, counter_name, value, duration, dispatch_id from counters_collection")
+        per = collections.defaultdict(lambda: collections.defaultdict(list))
+        dur = collections.defaultdict(dict)
+        for k, cn, v, d, disp in rows:
+            short = k.replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0]
+            per[short][cn].append(v)
+            dur[short][disp] = d
+        for k in per:
+            e = out.setdefault(k, {})
+            for cn, vals in per[k].items():
+                e[cn] = statistics.median(vals)
+            e.setdefault("dur_us", statistics.median(dur[k].values()) / 1e3)
+            e.setdefault("dispatches", len(dur[k]))
+    print(json.dumps(out, indent=1))
+
+
 def main():
     args = parse()
+    if args.summarize_sq:
+        summarize_sq(args.summarize_sq)
+        return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
         return
