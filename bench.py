@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run 2 steps of the workload plus a 256 MiB calibration copy and exit "
                          "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--pmc-batch", action="store_true",
+                    help="run one batched reintegration of 64 keyframes and the same 128 plain calls, and exit "
+                         "(for rocprofv3 --pmc)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="turn two rocprofv3 counter_collection.csv files of --pmc-workload runs into "
                          "profiles/pmc_traffic.json and profiles/pmc_traffic_room_large.txt")
@@ -729,6 +732,45 @@ def pmc_workload(args):
     print("[bench] pmc workload done", file=sys.stderr, flush=True)
 
 
+def pmc_batch(args):
+    """One batched reintegration of 64 keyframes (the batched_reintegration
+    workload at the default tuning) and the same 128 plain calls."""
+    import torch
+    from superscanner_amd import synth
+    from superscanner_amd.voxel_grid import GPUVoxelGrid
+    cfg = dict(synth.ROOM_LARGE)
+    poses64 = synth.traj_random_walk(n=args.frames, seed=2)
+    prims = synth.scene_room(seed=1)
+    rng = np.random.default_rng(21)
+    n = len(poses64)
+    kf = np.linspace(0, n - 1, args.keyframes).astype(int)
+    keep = rng.uniform(0.3, 1.0, args.keyframes)
+    kdepth = synth.frames(prims, [poses64[k] for k in kf], cfg, seed=31, keep=keep)
+    old = np.stack([synth.pose_f32(poses64[k]) for k in kf])
+    new = np.stack([synth.pose_f32(synth.perturb(poses64[k], rng)) for k in kf])
+    dev = torch.device("cuda", 0)
+    kd = torch.from_numpy(kdepth.view(np.int16)).to(dev)
+    grids = []
+    for _ in range(2):
+        g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], cfg["H"], cfg["W"],
+                         n_buckets=cfg["n_buckets"], n_bricks=1 << 18, max_weight=cfg["max_weight"])
+        g.setIntrinsic(*cfg["intr"])
+        for k in range(args.keyframes):
+            g.integrate(None, kd[k], old[k])
+        grids.append(g)
+    a, b = grids
+    b.reserve_batch(2 * args.keyframes)
+    torch.cuda.synchronize()
+    b.reintegrate_batch([kd[k] for k in range(args.keyframes)], old, new)
+    for k in range(args.keyframes):
+        a.deIntegrate(None, kd[k], old[k])
+        a.integrate(None, kd[k], new[k])
+    torch.cuda.synchronize()
+    for g in grids:
+        g.close()
+    print("[bench] pmc batch done", file=sys.stderr, flush=True)
+
+
 def summarize_pmc(fetch_csv, write_csv):
     """HBM bytes per launch of each kernel from FETCH_SIZE / WRITE_SIZE (KiB
     per dispatch), each scaled so that the calibration copy reads exactly its
@@ -826,6 +868,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     if args.pmc_workload:
         pmc_workload(args)
+        return
+    if args.pmc_batch:
+        pmc_batch(args)
         return
     dev = torch.device("cuda", 0)
     frames = args.frames

@@ -10,7 +10,10 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtsdf_hip.so")
+# TSDF_HIP_LIB names another build of the same sources (``make -C
+# superscanner_amd variant``) for timing a compile-time alternative; unset,
+# the in-tree library is loaded.
+LIB_PATH = os.environ.get("TSDF_HIP_LIB") or os.path.join(_HERE, "libtsdf_hip.so")
 
 TSDF_OK = 0
 TSDF_BAD_ARG = 1

@@ -44,6 +44,10 @@ namespace {
 
 
 
+#ifndef TSDF_DEPTH_GATHER
+#define TSDF_DEPTH_GATHER 0  // plain update: 1 gathers the depth and rebuilds the point, 0 gathers the point map
+#endif
+
 // One integrate (+1) or deintegrate (-1) operation.
 struct OpDesc {
     const uint16_t* depth;
@@ -387,7 +391,8 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
                             }
                 }
             }
-            op.ptmap[(size_t)i * g.W + j] = o;
+            if (kBatch || !TSDF_DEPTH_GATHER)
+                op.ptmap[(size_t)i * g.W + j] = o;
         }
         if (__ballot(box) == 0ull)
             continue;
@@ -536,6 +541,20 @@ __device__ __forceinline__ int project_voxel(const GridView& g, float cx, float 
     return ((unsigned)i < (unsigned)g.H && (unsigned)j < (unsigned)g.W) ? i * g.W + j : -1;
 }
 
+// project_voxel with the row and column: false when outside the image.
+__device__ __forceinline__ bool project_voxel_ij(const GridView& g, float cx, float cy, float cz, int& i, int& j)
+{
+    const float r = __builtin_amdgcn_rcpf(cz);
+    const float ny = cy * g.cam.fy, nx = cx * g.cam.fx;
+    const bool ok = ((int)rcp_certified(r, cz) & (int)pix_axis_fast(ny, r, g.cam.ci, i) &
+                     (int)pix_axis_fast(nx, r, g.cam.cj, j)) != 0;
+    if (!ok) {
+        i = pix_axis_ref(ny, cz, g.cam.ci);
+        j = pix_axis_ref(nx, cz, g.cam.cj);
+    }
+    return (unsigned)i < (unsigned)g.H && (unsigned)j < (unsigned)g.W;
+}
+
 // The voxels (x, y, 0..7) of one lane, held in registers across ops.
 template <bool kColor>
 struct LaneVoxels {
@@ -637,9 +656,58 @@ __device__ __forceinline__ unsigned int op_band(const GridView& g, const OpDesc&
     return band;
 }
 
+// op_band with the depth gathered instead of the point map: the world point
+// of pixel (i, j) is rebuilt as getPoint and transformPoint make it
+// (GPUVoxelGrid.h:229-232, 240-247): d = (float)depth / 1000 (RN32 through
+// one double product, quot_rn), x = colfac[j] * d, y = rowfac[i] * d,
+// z = d, then E; a pixel with d <= rmin is no measurement (the reference's
+// `vpt.z > rmin`).  pix is i * W + j, or -1.
+constexpr double kRmm = 0.001;  // RN64(1 / 1000)
+template <int kN>
+__device__ __forceinline__ unsigned int op_band_depth(const GridView& g, const OpDesc& op, float vmx, float vmy,
+                                                      const float (&vmz)[kVPE], int h, float (&sdf)[kVPE],
+                                                      int (&pix)[kVPE])
+{
+    const float* Ei = op.pose.Ei;
+    const float* E = op.pose.E;
+    const float a0 = vmx * Ei[0] + vmy * Ei[1];
+    const float a1 = vmx * Ei[4] + vmy * Ei[5];
+    const float a2 = vmx * Ei[8] + vmy * Ei[9];
+    const float ez0 = op.pose.Ez[0], ez1 = op.pose.Ez[1], ez2 = op.pose.Ez[2];
+    int ii[kVPE], jj[kVPE];
+    bool in[kVPE];
+#pragma unroll
+    for (int z = h; z < h + kN; ++z)
+        in[z] = project_voxel_ij(g, a0 + vmz[z] * Ei[2] + Ei[3], a1 + vmz[z] * Ei[6] + Ei[7],
+                                 a2 + vmz[z] * Ei[10] + Ei[11], ii[z], jj[z]);
+    uint16_t u[kVPE];
+    float cf[kVPE], rf[kVPE];
+#pragma unroll
+    for (int z = h; z < h + kN; ++z) {
+        const int i = in[z] ? ii[z] : 0, j = in[z] ? jj[z] : 0;
+        u[z] = op.depth[(size_t)i * op.pitch + j];
+        cf[z] = g.colfac[j];
+        rf[z] = g.rowfac[i];
+    }
+    unsigned int band = 0;
+#pragma unroll
+    for (int z = h; z < h + kN; ++z) {
+        const float d = quot_rn((float)u[z], kRmm);
+        f3 p;
+        p.x = cf[z] * d;
+        p.y = rf[z] * d;
+        p.z = d;
+        const f3 pt = xform(p, E);
+        sdf[z] = (pt.x - vmx) * ez0 + (pt.y - vmy) * ez1 + (pt.z - vmz[z]) * ez2;
+        pix[z] = in[z] ? ii[z] * g.W + jj[z] : -1;
+        band |= (in[z] && d > g.rmin && fabsf(sdf[z]) < g.trunc) ? 1u << z : 0u;
+    }
+    return band;
+}
+
 // The op applied to a lane's eight voxels held in registers, in two halves
 // of kHalf voxels.  Returns a bit per voxel that was updated.
-template <bool kColor, int kSign, int kHalf = 4>
+template <bool kColor, int kSign, bool kDepth = false, int kHalf = 4>
 __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc& op, float vmx, float vmy,
                                                  const float (&vmz)[kVPE], LaneVoxels<kColor>& v)
 {
@@ -648,7 +716,8 @@ __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc
     for (int h = 0; h < kVPE; h += kHalf) {
         float sdf[kVPE];
         int pix[kVPE];
-        const unsigned int band = op_band<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix);
+        const unsigned int band = kDepth ? op_band_depth<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix)
+                                         : op_band<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix);
 #pragma unroll
         for (int z = h; z < h + kHalf; ++z)
             if ((band >> z) & 1u)
@@ -846,7 +915,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, i
             LaneVoxels<kColor> v;
             load_lane<T, kColor>(g, id, lane, fresh, v);
             const LaneMid mid = lane_mid(g, off, m.y, m.z, m.w, lane);
-            const unsigned int upd = apply_op<kColor, kSign>(g, op, mid.x, mid.y, mid.z, v);
+            const unsigned int upd = apply_op<kColor, kSign, TSDF_DEPTH_GATHER != 0>(g, op, mid.x, mid.y, mid.z, v);
             nupd += __popc(upd);
             store_lane<T, kColor>(g, id, lane, fresh ? 0xFFu : upd, v);
         }
@@ -1414,6 +1483,8 @@ struct tsdf_grid {
     uint8_t* sbdf_tmp;
     SbdfInfo* sbdf_info;
     float4* ptmap;
+    float* colfac;  // [W] getPoint's column factors ((float)j - cj) / fx
+    float* rowfac;  // [H] getPoint's row factors ((float)i - ci) / fy
     Ctrl* ctrl;
     Ctrl* ctrl_host;  // pinned
     unsigned long long* stat;       // statistics counters, kStatWords
@@ -1521,6 +1592,8 @@ GridView view(const tsdf_grid* g)
     v.H = g->p.height;
     v.W = g->p.width;
     v.cam = g->cam;
+    v.colfac = g->colfac;
+    v.rowfac = g->rowfac;
     return v;
 }
 
@@ -1943,6 +2016,8 @@ tsdf_status tsdf_create(const tsdf_params* p, tsdf_grid** out)
     alloc((void**)&g->sbdf_tmp, kSbdfCap);
     alloc((void**)&g->sbdf_info, sizeof(SbdfInfo));
     alloc((void**)&g->ptmap, (size_t)p->height * p->width * sizeof(float4));
+    alloc((void**)&g->colfac, (size_t)p->width * sizeof(float));
+    alloc((void**)&g->rowfac, (size_t)p->height * sizeof(float));
     alloc((void**)&g->ctrl, sizeof(Ctrl));
     alloc((void**)&g->stat, kStatWords * sizeof(unsigned long long));
     {
@@ -2012,6 +2087,8 @@ void tsdf_destroy(tsdf_grid* g)
     (void)hipFree(g->sbdf_tmp);
     (void)hipFree(g->sbdf_info);
     (void)hipFree(g->ptmap);
+    (void)hipFree(g->colfac);
+    (void)hipFree(g->rowfac);
     (void)hipFree(g->ctrl);
     (void)hipFree(g->stat);
     (void)hipFree(g->ray_cost);
@@ -2085,6 +2162,18 @@ tsdf_status tsdf_set_intrinsics(tsdf_grid* g, float fx, float fy, float cx, floa
     g->cam.fy = fy;
     g->cam.ci = (float)g->p.height - cy;  // GPUVoxelGrid.cpp:259
     g->cam.cj = cx;                       // GPUVoxelGrid.cpp:260
+    // getPoint's first factors per column and row (binary32 on the host, the
+    // same IEEE division the device makes), for the depth-gather update
+    {
+        std::vector<float> cf((size_t)g->p.width), rf((size_t)g->p.height);
+        for (int j = 0; j < g->p.width; ++j)
+            cf[j] = ((float)j - g->cam.cj) / g->cam.fx;
+        for (int i = 0; i < g->p.height; ++i)
+            rf[i] = ((float)i - g->cam.ci) / g->cam.fy;
+        DeviceScope ds(g->p.device);
+        HIPC(hipMemcpy(g->colfac, cf.data(), cf.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(g->rowfac, rf.data(), rf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     g->intr_set = true;
     return TSDF_OK;
 }

@@ -81,6 +81,8 @@ struct GridView {
     double Rvbl;   // RN64(1 / vbl): exact block index through quot_rn
     int H, W;
     Camera cam;
+    const float* colfac;  // [W] ((float)j - cj) / fx, the first factor of getPoint's x
+    const float* rowfac;  // [H] ((float)i - ci) / fy, the first factor of getPoint's y
 };
 
 // Brick payload: 512 (sdf, w) pairs, voxel v = x + 8y + 64z at pair v, so a

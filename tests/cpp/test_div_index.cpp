@@ -107,6 +107,16 @@ int main(int argc, char** argv)
         }
         std::printf("sweep: %ld quotients equal to the IEEE division\n", swept);
     }
+    // every uint16 depth: quot_rn(u, RN64(1 / 1000)) is the reference's
+    // (float)u / 1000.0f (getPoint's metres, the depth-gather update)
+    for (uint32_t u = 0; u <= 0xFFFFu; ++u) {
+        const float a = (float)u;
+        if (tsdf::quot_rn(a, 1.0 / 1000.0) != a / 1000.0f) {
+            std::printf("DEPTH MISMATCH u=%u\n", u);
+            std::exit(1);
+        }
+    }
+    std::printf("depth: 65536 metre values equal to the IEEE division\n");
     std::printf("div_index: %ld inputs agree with the reference arithmetic (%ld took the exact path)\n",
                 checked, slow);
     return 0;

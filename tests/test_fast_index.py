@@ -25,6 +25,7 @@ def test_div_index_matches_reference(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "agree with the reference arithmetic" in r.stdout
     assert "quotients equal to the IEEE division" in r.stdout
+    assert "65536 metre values equal to the IEEE division" in r.stdout
 
 
 @pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
