@@ -217,7 +217,8 @@ tsdf_status tsdf_raycast(tsdf_grid *g, const float E_cam2world[16], float *point
  * buffers are DEVICE [n_views][h][w]; depth is camera-frame z in metres,
  * 0 where no surface is found.  sdf: optional DEVICE [n_bricks][512] float
  * values that replace the stored sdf (brick order of tsdf_export_bricks),
- * NULL = the stored values; weights always come from the map. */
+ * NULL = the stored values (an override needs a grid of at most 2^23 - 1
+ * bricks); weights always come from the map. */
 
 /* normals: DEVICE [n][h][w][3] unit gradient of the field, or NULL. */
 tsdf_status tsdf_render(tsdf_grid *g, int32_t n_views, const float *E, const tsdf_view *cam,
@@ -226,7 +227,12 @@ tsdf_status tsdf_render(tsdf_grid *g, int32_t n_views, const float *E, const tsd
 /* Adjoint of tsdf_render for a rendered `depth` and upstream gradient
  * grad_depth = dL/dD: grad_pose DEVICE [n][6] (overwritten) = dL/dxi for the
  * left perturbation E' = exp(xi^) E, xi = (translation, rotation);
- * grad_sdf DEVICE [n_bricks][512] (accumulated) = dL/dV, or NULL. */
+ * grad_sdf DEVICE [n_bricks][512] (accumulated) = dL/dV, or NULL.  Sums over
+ * pixels are taken in a fixed order: a repeated call gives the same bits.
+ * The sdf override and grad_sdf address at most 2^23 - 1 bricks, and
+ * grad_sdf at most 2^31 - 1 records (8 per pixel of all views):
+ * TSDF_BAD_ARG beyond.  Grows grid-owned scratch (fp32 partials per
+ * 16x16 pixels; with grad_sdf, 128 B of records per pixel) on first need. */
 tsdf_status tsdf_render_backward(tsdf_grid *g, int32_t n_views, const float *E, const tsdf_view *cam,
                                  const float *sdf, const float *depth, const float *grad_depth,
                                  float *grad_pose, float *grad_sdf, void *stream);

@@ -19,6 +19,7 @@
 // leaves the device: k_integrate is a persistent grid that reads it.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstddef>
@@ -1528,6 +1529,16 @@ struct tsdf_grid {
     ViewPose* view_host;  // pinned
     hipEvent_t render_done;
     bool render_staged;
+    // fixed-order reductions of the render adjoint and the tracking sums,
+    // grown on demand: per-workgroup partial sums, the (voxel, value)
+    // records of grad_sdf with their sorted copies, the sort's temporary
+    float* red_part;
+    size_t red_part_cap;  // floats
+    uint32_t* rec_key[2];
+    float* rec_val[2];
+    size_t rec_cap;  // records
+    void* sort_tmp;
+    size_t sort_tmp_cap;  // bytes
 };
 
 namespace {
@@ -2119,6 +2130,12 @@ void tsdf_destroy(tsdf_grid* g)
     if (g->render_done)
         (void)hipEventDestroy(g->render_done);
     (void)hipFree(g->view_dev);
+    (void)hipFree(g->red_part);
+    for (int k = 0; k < 2; ++k) {
+        (void)hipFree(g->rec_key[k]);
+        (void)hipFree(g->rec_val[k]);
+    }
+    (void)hipFree(g->sort_tmp);
     if (g->stage_host)
         (void)hipHostFree(g->stage_host);
     if (g->staging_done)

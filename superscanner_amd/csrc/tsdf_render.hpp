@@ -22,6 +22,15 @@
 //           perturbation T' = exp(xi^) T, xi = (v, w):
 //             dD/dv = -grad phi / s, dD/dw = -(p* x grad phi) / s,
 //           and dD/dV_c = -a_c / s for the 8 voxels (trilinear weights a_c).
+//
+// Every sum over pixels is taken in a fixed order, so that a call repeated on
+// the same inputs gives the same bits: a workgroup sums its pixels (a fixed
+// shuffle tree, then its wavefronts in order) into its own partial; one
+// workgroup per (view, value) sums the partials in fp64, thread t taking
+// workgroups t, t + 256, ..., then the same tree.  grad_sdf is a scatter: each
+// pixel writes its 8 (voxel index, dL/dV) records at fixed positions, a
+// stable radix sort by voxel index keeps the pixel order within a voxel, and
+// one thread per voxel adds its run in that order.
 #pragma once
 
 namespace {
@@ -33,6 +42,9 @@ constexpr float kRMinStepVox = 0.5f;
 constexpr float kRGraze = 0.05f;
 constexpr int kRootIters = 16;
 constexpr int kNormalEq = 29;  // per view: JtJ upper triangle (21), Jtr (6), sum r^2, count
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;  // grad_sdf record of no voxel (sorts last)
+// voxel indices brick * 512 + v stay below kNoRecord for brick ids below this
+constexpr uint32_t kMaxIndexedBricks = (1u << 23) - 1u;
 
 struct RenderCam {
     int h, w;
@@ -61,7 +73,8 @@ __device__ __forceinline__ int cached_brick(const GridView& g, BrickCache& c, in
 struct Sample {
     float phi;
     float grad[3];
-    int idx[8];   // brick * 512 + v of the 8 voxels (corner c = (c&1, c>>1&1, c>>2))
+    uint32_t idx[8];  // brick * 512 + v of the 8 voxels (corner c = (c&1, c>>1&1, c>>2)),
+                      // meaningful for brick ids below kMaxIndexedBricks
     float wt[8];  // trilinear weights
 };
 
@@ -101,7 +114,7 @@ __device__ __forceinline__ bool field_sample(const GridView& g, const float* sdf
     for (int c = 0; c < 8; ++c) {
         const int ax = c & 1, ay = (c >> 1) & 1, az = (c >> 2) & 1;
         const int v = ((kx + ax) & 7) + 8 * ((ky + ay) & 7) + 64 * ((kz + az) & 7);
-        const int id = bid[c] * kVPB + v;
+        const uint32_t id = (uint32_t)bid[c] * (uint32_t)kVPB + (uint32_t)v;
         float s, w;
         ld_pair(pay + (size_t)bid[c] * (2 * kVPB), v, s, w);
         if (!(w > 0.0f))
@@ -234,6 +247,9 @@ struct RenderArgs {
     const float* grad_depth;  // backward: dL/dD [n][h][w]
     float* grad_pose;         // backward: [n][6]
     float* grad_sdf;          // backward: [n_bricks][512] or null
+    float* part;              // per-workgroup partial sums [n][workgroups][values]
+    uint32_t* rec_key;        // backward with grad_sdf: [n][h][w][8] voxel index or kNoRecord
+    float* rec_val;           // backward with grad_sdf: [n][h][w][8] dL/dV
     const float* depth_obs;   // tracking: observed depth [n][h][w] (m, 0 = none)
     float max_residual;
     double* normal_eq;        // tracking: [n][kNormalEq]
@@ -332,6 +348,13 @@ __global__ __launch_bounds__(kThreads) void k_render_backward(GridView g, Render
         const size_t u = ((size_t)view * a.cam.h + i) * a.cam.w + j;
         const float lam = a.depth[u];
         const float gd = a.grad_depth[u];
+        uint32_t key[8];
+        float val[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            key[c] = kNoRecord;
+            val[c] = 0.f;
+        }
         if (lam > 0.f && gd != 0.f) {
             const Ray r = make_ray(a.poses[view], a.cam, i, j);
             Sample smp;
@@ -340,18 +363,70 @@ __global__ __launch_bounds__(kThreads) void k_render_backward(GridView g, Render
 #pragma unroll
                 for (int k = 0; k < 6; ++k)
                     acc[k] = gd * J[k];
-                if (a.grad_sdf)
 #pragma unroll
-                    for (int c = 0; c < 8; ++c)
-                        atomicAdd(&a.grad_sdf[smp.idx[c]], gd * (-smp.wt[c] / s));
+                for (int c = 0; c < 8; ++c) {
+                    key[c] = smp.idx[c];
+                    val[c] = gd * (-smp.wt[c] / s);
+                }
             }
+        }
+        if (a.rec_key) {
+            uint4* kq = (uint4*)(a.rec_key + u * 8);
+            float4* vq = (float4*)(a.rec_val + u * 8);
+            kq[0] = make_uint4(key[0], key[1], key[2], key[3]);
+            kq[1] = make_uint4(key[4], key[5], key[6], key[7]);
+            vq[0] = make_float4(val[0], val[1], val[2], val[3]);
+            vq[1] = make_float4(val[4], val[5], val[6], val[7]);
         }
     }
     block_sum<6>(acc, lds);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
+        const size_t wg = ((size_t)view * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 #pragma unroll
         for (int k = 0; k < 6; ++k)
-            atomicAdd(&a.grad_pose[view * 6 + k], acc[k]);
+            a.part[wg * 6 + k] = acc[k];
+    }
+}
+
+// out[view][k] = the sum of the per-workgroup partials part[view][w][k] over
+// the nwg workgroups of a view, in fp64, in a fixed order.  grid = (nval,
+// n_views), one workgroup each.
+template <typename Out>
+__global__ __launch_bounds__(kThreads) void k_reduce_parts(const float* part, int nwg, int nval, Out* out)
+{
+    __shared__ double lds[kThreads / 64];
+    const int k = blockIdx.x, view = blockIdx.y;
+    double s = 0.0;
+    for (int w = threadIdx.x; w < nwg; w += kThreads)
+        s += (double)part[((size_t)view * nwg + w) * nval + k];
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        lds[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = lds[0];
+        for (int w = 1; w < kThreads / 64; ++w)
+            t += lds[w];
+        out[(size_t)view * nval + k] = (Out)t;
+    }
+}
+
+// grad[key] += the values of key's run in the sorted records, summed in fp64
+// in record order (pixel order, the sort being stable); one thread per run.
+__global__ __launch_bounds__(kThreads) void k_segment_add(const uint32_t* key, const float* val, size_t n,
+                                                          float* grad)
+{
+    for (size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x; p < n; p += (size_t)gridDim.x * kThreads) {
+        const uint32_t k = key[p];
+        if (k == kNoRecord || (p > 0 && key[p - 1] == k))
+            continue;
+        double t = 0.0;
+        for (size_t q = p; q < n && key[q] == k; ++q)
+            t += (double)val[q];
+        grad[k] = (float)((double)grad[k] + t);
+    }
 }
 
 // Gauss-Newton normal equations of depth tracking: for every pixel where
@@ -394,17 +469,12 @@ __global__ __launch_bounds__(kThreads) void k_track(GridView g, RenderArgs a)
         }
     }
     block_sum<kNormalEq>(acc, lds);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
+        const size_t wg = ((size_t)view * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 #pragma unroll
         for (int k = 0; k < kNormalEq; ++k)
-            if (acc[k] != 0.f)
-                atomicAdd(&a.normal_eq[view * kNormalEq + k], (double)acc[k]);
-}
-
-__global__ void k_fill_f32(float* p, size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 0.f;
+            a.part[wg * kNormalEq + k] = acc[k];
+    }
 }
 
 __global__ void k_fill_f64(double* p, size_t n)
@@ -457,6 +527,45 @@ tsdf_status render_setup(tsdf_grid* g, int32_t n_views, const float* E, const ts
     a.cam.rmin = g->p.min_depth;
     a.cam.dmax = g->p.max_ray_length;
     grid = dim3((cam->width + 15) / 16, (cam->height + 15) / 16, n_views);
+    return TSDF_OK;
+}
+
+// Grows the grid's reduction scratch to at least `part` partial sums,
+// `rec` grad_sdf records (two copies: the sort's input and output) and `tmp`
+// bytes of sort temporary.  Calls that overlap on one grid share it.
+tsdf_status ensure_reduce(tsdf_grid* g, size_t part, size_t rec, size_t tmp)
+{
+    if (g->red_part_cap >= part && g->rec_cap >= rec && g->sort_tmp_cap >= tmp)
+        return TSDF_OK;
+    HIPC(hipDeviceSynchronize());  // work queued before may still use the old buffers
+    if (g->red_part_cap < part) {
+        (void)hipFree(g->red_part);
+        g->red_part = nullptr;
+        g->red_part_cap = 0;
+        HIPC(hipMalloc(&g->red_part, part * sizeof(float)));
+        g->red_part_cap = part;
+    }
+    if (g->rec_cap < rec) {
+        for (int k = 0; k < 2; ++k) {
+            (void)hipFree(g->rec_key[k]);
+            (void)hipFree(g->rec_val[k]);
+            g->rec_key[k] = nullptr;
+            g->rec_val[k] = nullptr;
+        }
+        g->rec_cap = 0;
+        for (int k = 0; k < 2; ++k) {
+            HIPC(hipMalloc(&g->rec_key[k], rec * sizeof(uint32_t)));
+            HIPC(hipMalloc(&g->rec_val[k], rec * sizeof(float)));
+        }
+        g->rec_cap = rec;
+    }
+    if (g->sort_tmp_cap < tmp) {
+        (void)hipFree(g->sort_tmp);
+        g->sort_tmp = nullptr;
+        g->sort_tmp_cap = 0;
+        HIPC(hipMalloc(&g->sort_tmp, tmp));
+        g->sort_tmp_cap = tmp;
+    }
     return TSDF_OK;
 }
 
@@ -572,6 +681,8 @@ tsdf_status tsdf_render(tsdf_grid* g, int32_t n_views, const float* E, const tsd
     tsdf_status st = render_setup(g, n_views, E, cam, s, a, grid);
     if (st != TSDF_OK || n_views == 0)
         return st;
+    if (sdf && g->p.n_bricks > kMaxIndexedBricks)
+        return fail(TSDF_BAD_ARG, "an sdf override addresses at most 2^23 - 1 bricks");
     a.sdf = sdf;
     a.depth = depth;
     a.normals = normals;
@@ -596,17 +707,47 @@ tsdf_status tsdf_render_backward(tsdf_grid* g, int32_t n_views, const float* E, 
     tsdf_status st = render_setup(g, n_views, E, cam, s, a, grid);
     if (st != TSDF_OK || n_views == 0)
         return st;
+    if ((sdf || grad_sdf) && g->p.n_bricks > kMaxIndexedBricks)
+        return fail(TSDF_BAD_ARG, "an sdf override or grad_sdf addresses at most 2^23 - 1 bricks");
+    const size_t nwg = (size_t)grid.x * grid.y;
+    const size_t n_rec = grad_sdf ? (size_t)n_views * cam->height * cam->width * 8 : 0;
+    if (n_rec > (size_t)0x7FFFFFFF)
+        return fail(TSDF_BAD_ARG, "grad_sdf over more than 2^31 - 1 records (8 per pixel of all views)");
+    // radix bits: kNoRecord's low bits sort after every voxel index
+    int bits = 1;
+    while (((1ull << bits) - 1ull) < (unsigned long long)g->p.n_bricks * kVPB)
+        ++bits;
+    size_t tmp = 0;
+    if (n_rec)
+        HIPC(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, g->rec_key[0], g->rec_key[1], g->rec_val[0],
+                                                g->rec_val[1], (int)n_rec, 0, bits, s));
+    st = ensure_reduce(g, nwg * n_views * 6, n_rec, tmp);
+    if (st != TSDF_OK)
+        return st;
     a.sdf = sdf;
     a.depth = (float*)depth;
     a.grad_depth = grad_depth;
     a.grad_pose = grad_pose;
     a.grad_sdf = grad_sdf;
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(kThreads), 0, s, grad_pose, (size_t)n_views * 6);
+    a.part = g->red_part;
+    a.rec_key = n_rec ? g->rec_key[0] : nullptr;
+    a.rec_val = n_rec ? g->rec_val[0] : nullptr;
     if (g->p.storage == TSDF_STORAGE_F16)
         hipLaunchKernelGGL(k_render_backward<__half>, grid, dim3(kThreads), 0, s, view(g), a);
     else
         hipLaunchKernelGGL(k_render_backward<float>, grid, dim3(kThreads), 0, s, view(g), a);
     HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_parts<float>, dim3(6, n_views), dim3(kThreads), 0, s, g->red_part, (int)nwg, 6,
+                       grad_pose);
+    HIPC(hipGetLastError());
+    if (n_rec) {
+        HIPC(hipcub::DeviceRadixSort::SortPairs(g->sort_tmp, tmp, g->rec_key[0], g->rec_key[1], g->rec_val[0],
+                                                g->rec_val[1], (int)n_rec, 0, bits, s));
+        const size_t nb = std::min((n_rec + kThreads - 1) / kThreads, (size_t)g->n_cu * 16);
+        hipLaunchKernelGGL(k_segment_add, dim3((unsigned)nb), dim3(kThreads), 0, s, g->rec_key[1], g->rec_val[1],
+                           n_rec, grad_sdf);
+        HIPC(hipGetLastError());
+    }
     return TSDF_OK;
 }
 
@@ -625,15 +766,22 @@ tsdf_status tsdf_track_normal_eq(tsdf_grid* g, int32_t n_views, const float* E, 
     tsdf_status st = render_setup(g, n_views, E, cam, s, a, grid);
     if (st != TSDF_OK || n_views == 0)
         return st;
+    const size_t nwg = (size_t)grid.x * grid.y;
+    st = ensure_reduce(g, nwg * n_views * kNormalEq, 0, 0);
+    if (st != TSDF_OK)
+        return st;
     a.depth = depth;
     a.depth_obs = depth_obs;
     a.max_residual = max_residual;
     a.normal_eq = normal_eq;
-    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(kThreads), 0, s, normal_eq, (size_t)n_views * kNormalEq);
+    a.part = g->red_part;
     if (g->p.storage == TSDF_STORAGE_F16)
         hipLaunchKernelGGL(k_track<__half>, grid, dim3(kThreads), 0, s, view(g), a);
     else
         hipLaunchKernelGGL(k_track<float>, grid, dim3(kThreads), 0, s, view(g), a);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_parts<double>, dim3(kNormalEq, n_views), dim3(kThreads), 0, s, g->red_part,
+                       (int)nwg, kNormalEq, normal_eq);
     HIPC(hipGetLastError());
     return TSDF_OK;
 }

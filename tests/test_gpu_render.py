@@ -110,6 +110,41 @@ def test_backward_matches_oracle(gpu):
     assert np.abs(gs[exp_sdf == 0]).sum() < 1e-3 * np.abs(gs).sum()
 
 
+def test_backward_and_tracking_repeat_bit_for_bit(gpu):
+    """Every pixel sum is taken in a fixed order (per-workgroup partials
+    summed in fp64 in workgroup order; grad_sdf records sorted stably by
+    voxel and added run by run), so repeated calls give identical bits."""
+    g, f, ex, poses, cfg = _map()
+    T = _views(poses, 3, seed=5)
+    vi = _vintr(cfg)
+    dep = dr.render(g, T, VH, VW, vi)
+    gdt = torch.from_numpy(np.random.default_rng(4).normal(size=dep.shape).astype(np.float32)).to(g.device)
+    gdt *= (dep > 0)
+    runs = [dr.render_backward(g, T, VH, VW, vi, dep, gdt, sdf=ex["sdf"].contiguous()) for _ in range(3)]
+    gp0, gs0 = (x.cpu().numpy() for x in runs[0])
+    assert np.abs(gp0).sum() > 0 and (gs0 != 0).sum() > 1000
+    for gp, gs in runs[1:]:
+        assert np.array_equal(gp.cpu().numpy().view(np.uint32), gp0.view(np.uint32))
+        assert np.array_equal(gs.cpu().numpy().view(np.uint32), gs0.view(np.uint32))
+    # without grad_sdf the pose gradient is the same
+    gp_only, none = dr.render_backward(g, T, VH, VW, vi, dep, gdt, sdf=ex["sdf"].contiguous(), want_sdf=False)
+    assert none is None and np.array_equal(gp_only.cpu().numpy().view(np.uint32), gp0.view(np.uint32))
+    import ctypes as C
+    E = dr._poses_f32(T)
+    cam = dr.view_of(VH, VW, vi)
+    obs = dep * 1.001
+    outs = []
+    for _ in range(3):
+        ne = torch.empty((3, 29), dtype=torch.float64, device=g.device)
+        st = g._L.tsdf_track_normal_eq(g._h, 3, dr._fptr(E), C.byref(cam), dr._ptr(obs), 0.05, dr._ptr(ne),
+                                       None, dr._stream())
+        assert st == 0
+        outs.append(ne.cpu().numpy())
+    assert outs[0][:, 28].min() > 1000
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
+
+
 def test_autograd_function_chain(gpu):
     """RenderDepth through torch.autograd at xi != 0: the pose gradient is
     J_l(xi)^T times the left-perturbation gradient, the sdf gradient lands on
