@@ -217,6 +217,19 @@ struct Footprint {
     unsigned long long* counts;  // [0] distinct keys, [1] set overflow, [2] distinct voxels
 };
 
+// Host builds (tests/cpp) may observe the loads of the kStats march: every
+// loaded hash lookup and every voxel read, in march order.
+struct HostTrace {
+    void (*key)(void* ctx, uint32_t key);
+    void (*voxel)(void* ctx, int id, int v);
+    void* ctx;
+};
+inline HostTrace*& host_trace()
+{
+    static HostTrace* t = nullptr;
+    return t;
+}
+
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline void fp_key(const Footprint& fp, uint32_t key)
 {
@@ -243,8 +256,16 @@ __device__ inline void fp_voxel(const Footprint& fp, int id, int v)
     atomicOr(&fp.vox_bits[(size_t)id * 16 + (v >> 5)], 1u << (v & 31));
 }
 #else
-inline void fp_key(const Footprint&, uint32_t) {}
-inline void fp_voxel(const Footprint&, int, int) {}
+inline void fp_key(const Footprint&, uint32_t key)
+{
+    if (HostTrace* t = host_trace())
+        t->key(t->ctx, key);
+}
+inline void fp_voxel(const Footprint&, int id, int v)
+{
+    if (HostTrace* t = host_trace())
+        t->voxel(t->ctx, id, v);
+}
 #endif
 
 // Raycast constants of one call.

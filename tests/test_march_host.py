@@ -30,6 +30,8 @@ def march_lib(tmp_path_factory):
     L.march_host.argtypes = [i, i, f, f, f, f, f, f, f, i, vp, vp, vp, vp, C.c_uint32, i, vp, vp, vp, vp,
                              C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int)]
     L.march_host.restype = C.c_int
+    L.march_tile_reuse.argtypes = [i, i, f, f, f, f, f, f, f, i, vp, vp, vp, vp, C.c_uint32, i, vp, vp]
+    L.march_tile_reuse.restype = C.c_int
     return L
 
 
@@ -96,3 +98,47 @@ def test_march_room(march_lib):
         if k == 12:
             o.integrate(depth[2], synth.pose_f32(poses[2]), -1)
     check(march_lib, o, cfg, intr, synth.pose_f32(poses[19]), H, W)
+
+
+def test_tile_reuse_of_lookups_and_voxels(march_lib):
+    """What a per-workgroup LDS cache could serve on the room workload
+    (848 x 480, a map of 40 frames, tile depth bounds on): the raycast replayed
+    tile by tile as the kernel groups its rays.  The figures are printed; the
+    assertions hold the shape of the result: a tile's loaded lookups repeat
+    (an ideal key cache would serve > 90 % of them), its voxel reads mostly do
+    not (< 40 %), and with every lane of a wavefront waiting for the slowest,
+    fewer than half of the wavefront lookup steps would be served whole."""
+    H, W = 480, 848
+    cfg = synth.ROOM_LARGE
+    intr = cfg["intr"]
+    poses = synth.traj_random_walk(n=40, seed=2)
+    depth = synth.frames(synth.scene_room(seed=1), poses, dict(cfg), seed=4)
+    oracle.set_threads(8)
+    try:
+        o = oracle.OracleGrid(cfg["voxel_length"], cfg["truncation"], H, W, max_weight=128, max_bricks=1 << 20)
+        o.set_intrinsics(*intr)
+        for k in range(40):
+            o.integrate(depth[k], synth.pose_f32(poses[k]))
+    finally:
+        oracle.set_threads(1)
+    ex = o.export()
+    n = len(ex["keys"])
+    nb = 1 << max(10, (2 * n).bit_length())
+    E = np.ascontiguousarray(synth.pose_f32(poses[39]), np.float32).reshape(16)
+    st = np.zeros(10, np.uint64)
+    f = C.c_float
+    assert march_lib.march_tile_reuse(H, W, f(intr[0]), f(intr[1]), f(intr[2]), f(intr[3]), f(cfg["voxel_length"]),
+                                      f(cfg["truncation"]), f(10.0), n, ex["keys"].ctypes.data,
+                                      ex["coords"].ctypes.data, ex["sdf"].ctypes.data, ex["w"].ctypes.data,
+                                      C.c_uint32(nb), 2, E.ctypes.data, st.ctypes.data) == 0
+    # the traced march is the march: same sample and probe counts
+    _, _, _, ns, npr = run_host(march_lib, o, cfg, intr, E, H, W, 2)
+    assert (int(st[8]), int(st[9])) == (ns, npr)
+    lookups, keys, vreads, voxels, lane_hits, steps, whole, kmax = (int(x) for x in st[:8])
+    ideal_key, ideal_vox = 1 - keys / lookups, 1 - voxels / vreads
+    print(f"\nloaded lookups {lookups}, distinct (tile, key) {keys}: ideal key-cache hit {ideal_key:.3f}; "
+          f"voxel reads {vreads}, distinct (tile, voxel) {voxels}: ideal voxel reuse {ideal_vox:.3f}; "
+          f"lockstep lane hits {lane_hits / lookups:.3f}, wavefront steps served whole {whole / steps:.3f}; "
+          f"most keys in one tile {kmax}")
+    assert ideal_key > 0.9 and ideal_vox < 0.4
+    assert whole / steps < 0.5 and kmax < 1024
