@@ -71,45 +71,51 @@ def tune_reintegration(make_grid, depths, old_poses, new_poses, *, ops_per_pass=
     return {"batch_ops_per_pass": per, "integrate_wg_per_cu": wg}, table
 
 
-def tune_integrate(grid, depths, poses, *, wg_per_cu=INTEGRATE_WG_PER_CU):
+def tune_integrate(grid, depths, poses, *, wg_per_cu=INTEGRATE_WG_PER_CU, rounds=3):
     """Picks integrate_wg_per_cu for plain integrate calls on ``grid``.  Each
     candidate integrates every frame and deintegrates it again, which leaves
     the weights as they were (the sdf values may differ in the last bits).
+    The candidates are timed in `rounds` interleaved rounds and each keeps
+    its best time, so one disturbed sample does not decide.
     Returns (best tuning dict, [(wg_per_cu, ms), ...])."""
-    table = []
-    for wg in wg_per_cu:
-        grid.set_tuning(integrate_wg_per_cu=wg)
+    best = {wg: float("inf") for wg in wg_per_cu}
+    for _ in range(rounds):
+        for wg in wg_per_cu:
+            grid.set_tuning(integrate_wg_per_cu=wg)
 
-        def run():
-            for d, E in zip(depths, poses):
-                grid.integrate(None, d, E)
-            for d, E in zip(depths, poses):
-                grid.deIntegrate(None, d, E)
+            def run():
+                for d, E in zip(depths, poses):
+                    grid.integrate(None, d, E)
+                for d, E in zip(depths, poses):
+                    grid.deIntegrate(None, d, E)
 
-        run()  # warm
-        table.append((wg, _timed(run)))
+            run()  # warm
+            best[wg] = min(best[wg], _timed(run))
+    table = [(wg, best[wg]) for wg in wg_per_cu]
     wg = min(table, key=lambda r: r[1])[0]
     grid.set_tuning(integrate_wg_per_cu=wg)
     return {"integrate_wg_per_cu": wg}, table
 
 
-def tune_raycast(grid, poses, *, xcd_map=(0,), skip=(0, 1, 2), order=(0, 2), budget=(0, 12, 16)):
+def tune_raycast(grid, poses, *, xcd_map=(0,), skip=(0, 1, 2), order=(0, 2), budget=(0, 12, 16), rounds=3):
     """Picks raycast_xcd_map, raycast_skip, raycast_order and raycast_budget
     for ``grid.raycast`` (the group march replays no superblock field, so
-    skip 1 runs without a budget).  Returns (best tuning dict, [(xcd_map,
-    skip, order, budget, ms), ...])."""
-    table = []
-    for xm, sk, od, bu in itertools.product(xcd_map, skip, order, budget):
-        if sk == 1 and bu:
-            continue
-        grid.set_tuning(raycast_xcd_map=xm, raycast_skip=sk, raycast_order=od, raycast_budget=bu)
+    skip 1 runs without a budget).  The candidates are timed in `rounds`
+    interleaved rounds and each keeps its best time.  Returns (best tuning
+    dict, [(xcd_map, skip, order, budget, ms), ...])."""
+    cands = [c for c in itertools.product(xcd_map, skip, order, budget) if not (c[1] == 1 and c[3])]
+    best = {c: float("inf") for c in cands}
 
-        def run():
-            for E in poses:
-                grid.raycast(E)
+    def run():
+        for E in poses:
+            grid.raycast(E)
 
-        run()  # warm (and, for order 2, the costs the first timed raycast orders by)
-        table.append((xm, sk, od, bu, _timed(run)))
+    for _ in range(rounds):
+        for xm, sk, od, bu in cands:
+            grid.set_tuning(raycast_xcd_map=xm, raycast_skip=sk, raycast_order=od, raycast_budget=bu)
+            run()  # warm (and, for order 2, the costs the first timed raycast orders by)
+            best[(xm, sk, od, bu)] = min(best[(xm, sk, od, bu)], _timed(run))
+    table = [(*c, best[c]) for c in cands]
     xm, sk, od, bu, _ = min(table, key=lambda r: r[4])
     grid.set_tuning(raycast_xcd_map=xm, raycast_skip=sk, raycast_order=od, raycast_budget=bu)
     return {"raycast_xcd_map": xm, "raycast_skip": sk, "raycast_order": od, "raycast_budget": bu}, table

@@ -48,6 +48,9 @@ namespace {
 #ifndef TSDF_DEPTH_GATHER
 #define TSDF_DEPTH_GATHER 0  // plain update: 1 gathers the depth and rebuilds the point, 0 gathers the point map
 #endif
+#ifndef TSDF_DEPTH_TILE
+#define TSDF_DEPTH_TILE 0  // plain update: 1 stages each brick's point-map rectangle in LDS
+#endif
 
 // One integrate (+1) or deintegrate (-1) operation.
 struct OpDesc {
@@ -706,19 +709,125 @@ __device__ __forceinline__ unsigned int op_band_depth(const GridView& g, const O
     return band;
 }
 
+// A brick's rectangle of the point map staged in LDS: rows i0 .. i0 + h - 1,
+// columns j0 .. j0 + w - 1, row-major; h = 0 when nothing is staged.
+constexpr int kTilePx = 320;  // 5 KiB of LDS per wavefront
+struct DepthTile {
+    int i0, j0, h, w;
+    const float4* px;
+};
+
+// Stages the point-map rectangle that brick (bx, by, bz) projects to.  Lanes
+// 0..7 project the brick's 8 corners (an approximate reciprocal: the
+// rectangle bounds the voxels' pixels with 2 pixels of margin, and a voxel
+// whose exact pixel still falls outside it reads the point map in global
+// memory, so no result depends on the rectangle).  One wavefront owns `lt`.
+[[maybe_unused]] __device__ __forceinline__ DepthTile stage_depth_tile(const GridView& g, const OpDesc& op, int bx, int by, int bz,
+                                                      int lane, float4* lt)
+{
+    const float* Ei = op.pose.Ei;
+    const int c = lane & 7;
+    const float px = g.vbl * (float)(bx + (c & 1)), py = g.vbl * (float)(by + ((c >> 1) & 1)),
+                pz = g.vbl * (float)(bz + (c >> 2));
+    const float cx = px * Ei[0] + py * Ei[1] + pz * Ei[2] + Ei[3];
+    const float cy = px * Ei[4] + py * Ei[5] + pz * Ei[6] + Ei[7];
+    const float cz = px * Ei[8] + py * Ei[9] + pz * Ei[10] + Ei[11];
+    const float r = __builtin_amdgcn_rcpf(cz);
+    // clamped to a range that converts to int exactly
+    const float lim = 65536.0f;
+    const float fi = fminf(fmaxf(cy * g.cam.fy * r + g.cam.ci, -lim), lim);
+    const float fj = fminf(fmaxf(cx * g.cam.fx * r + g.cam.cj, -lim), lim);
+    float imin = fi, imax = fi, jmin = fj, jmax = fj;
+    bool front = cz > 1e-3f;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        imin = fminf(imin, __shfl_xor(imin, m));
+        imax = fmaxf(imax, __shfl_xor(imax, m));
+        jmin = fminf(jmin, __shfl_xor(jmin, m));
+        jmax = fmaxf(jmax, __shfl_xor(jmax, m));
+        front = front && __shfl_xor((int)front, m);
+    }
+    DepthTile t;
+    t.px = lt;
+    const float ui0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(imin)));
+    const float ui1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(imax)));
+    const float uj0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(jmin)));
+    const float uj1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(jmax)));
+    const bool ufront = __builtin_amdgcn_readfirstlane((int)front) != 0;
+    t.i0 = max(0, (int)floorf(ui0) - 2);
+    t.j0 = max(0, (int)floorf(uj0) - 2);
+    const int i1 = min(g.H - 1, (int)floorf(ui1) + 2), j1 = min(g.W - 1, (int)floorf(uj1) + 2);
+    t.h = i1 - t.i0 + 1;
+    t.w = j1 - t.j0 + 1;
+    if (!ufront || t.h <= 0 || t.w <= 0 || t.h * t.w > kTilePx) {
+        t.h = 0;
+        t.w = 1;
+        return t;
+    }
+    const int area = t.h * t.w;
+    for (int k = lane; k < area; k += 64) {
+        const int row = k / t.w, col = k - row * t.w;
+        lt[k] = op.ptmap[(size_t)(t.i0 + row) * g.W + (t.j0 + col)];
+    }
+    __builtin_amdgcn_wave_barrier();
+    return t;
+}
+
+// op_band with the point map read from the brick's LDS rectangle where the
+// voxel's exact pixel lies in it, from global memory elsewhere.
+template <int kN>
+__device__ __forceinline__ unsigned int op_band_tile(const GridView& g, const OpDesc& op, float vmx, float vmy,
+                                                     const float (&vmz)[kVPE], int h, float (&sdf)[kVPE],
+                                                     int (&pix)[kVPE], const DepthTile& dt)
+{
+    const float* Ei = op.pose.Ei;
+    const float a0 = vmx * Ei[0] + vmy * Ei[1];
+    const float a1 = vmx * Ei[4] + vmy * Ei[5];
+    const float a2 = vmx * Ei[8] + vmy * Ei[9];
+    const float ez0 = op.pose.Ez[0], ez1 = op.pose.Ez[1], ez2 = op.pose.Ez[2];
+    int ii[kVPE], jj[kVPE];
+    bool in[kVPE];
+#pragma unroll
+    for (int z = h; z < h + kN; ++z)
+        in[z] = project_voxel_ij(g, a0 + vmz[z] * Ei[2] + Ei[3], a1 + vmz[z] * Ei[6] + Ei[7],
+                                 a2 + vmz[z] * Ei[10] + Ei[11], ii[z], jj[z]);
+    float4 q[kVPE];
+#pragma unroll
+    for (int z = h; z < h + kN; ++z) {
+        const unsigned int di = (unsigned int)ii[z] - (unsigned int)dt.i0;
+        const unsigned int dj = (unsigned int)jj[z] - (unsigned int)dt.j0;
+        const bool inr = in[z] && di < (unsigned int)dt.h && dj < (unsigned int)dt.w;
+        q[z] = dt.px[inr ? di * (unsigned int)dt.w + dj : 0u];
+        if (in[z] && !inr)
+            q[z] = op.ptmap[ii[z] * g.W + jj[z]];
+    }
+    unsigned int band = 0;
+#pragma unroll
+    for (int z = h; z < h + kN; ++z) {
+        sdf[z] = (q[z].x - vmx) * ez0 + (q[z].y - vmy) * ez1 + (q[z].z - vmz[z]) * ez2;
+        pix[z] = in[z] ? ii[z] * g.W + jj[z] : -1;
+        band |= (in[z] && fabsf(sdf[z]) < g.trunc) ? 1u << z : 0u;
+    }
+    return band;
+}
+
 // The op applied to a lane's eight voxels held in registers, in two halves
 // of kHalf voxels.  Returns a bit per voxel that was updated.
-template <bool kColor, int kSign, bool kDepth = false, int kHalf = 4>
+// kBand: 0 gathers the point map, 1 the depth (op_band_depth), 2 reads the
+// brick's LDS rectangle dt (op_band_tile).
+template <bool kColor, int kSign, int kBand = 0, int kHalf = 4>
 __device__ __forceinline__ unsigned int apply_op(const GridView& g, const OpDesc& op, float vmx, float vmy,
-                                                 const float (&vmz)[kVPE], LaneVoxels<kColor>& v)
+                                                 const float (&vmz)[kVPE], LaneVoxels<kColor>& v,
+                                                 const DepthTile& dt = DepthTile{0, 0, 0, 1, nullptr})
 {
     unsigned int upd = 0;
 #pragma unroll
     for (int h = 0; h < kVPE; h += kHalf) {
         float sdf[kVPE];
         int pix[kVPE];
-        const unsigned int band = kDepth ? op_band_depth<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix)
-                                         : op_band<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix);
+        const unsigned int band = kBand == 1   ? op_band_depth<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix)
+                                  : kBand == 2 ? op_band_tile<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix, dt)
+                                               : op_band<kHalf>(g, op, vmx, vmy, vmz, h, sdf, pix);
 #pragma unroll
         for (int z = h; z < h + kHalf; ++z)
             if ((band >> z) & 1u)
@@ -889,6 +998,9 @@ template <typename T, bool kColor, int kSign>
 __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, int par, Offsets off)
 {
     __shared__ int4 meta[kMetaChunk];
+#if TSDF_DEPTH_TILE
+    __shared__ float4 tiles[kThreads / 64][kTilePx];
+#endif
     const unsigned int count = g.ctrl->touched_count[par];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         g.ctrl->touched_count[par ^ 1] = 0u;  // ready for the next call
@@ -916,7 +1028,13 @@ __global__ __launch_bounds__(kThreads) void k_integrate(GridView g, OpDesc op, i
             LaneVoxels<kColor> v;
             load_lane<T, kColor>(g, id, lane, fresh, v);
             const LaneMid mid = lane_mid(g, off, m.y, m.z, m.w, lane);
-            const unsigned int upd = apply_op<kColor, kSign, TSDF_DEPTH_GATHER != 0>(g, op, mid.x, mid.y, mid.z, v);
+#if TSDF_DEPTH_TILE
+            const DepthTile dt = stage_depth_tile(g, op, m.y, m.z, m.w, lane, tiles[wave]);
+            const unsigned int upd = apply_op<kColor, kSign, 2>(g, op, mid.x, mid.y, mid.z, v, dt);
+#else
+            const unsigned int upd =
+                apply_op<kColor, kSign, TSDF_DEPTH_GATHER ? 1 : 0>(g, op, mid.x, mid.y, mid.z, v);
+#endif
             nupd += __popc(upd);
             store_lane<T, kColor>(g, id, lane, fresh ? 0xFFu : upd, v);
         }
