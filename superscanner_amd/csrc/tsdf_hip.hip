@@ -51,6 +51,9 @@ namespace {
 #ifndef TSDF_DEPTH_TILE
 #define TSDF_DEPTH_TILE 0  // plain update: 1 stages each brick's point-map rectangle in LDS
 #endif
+#ifndef TSDF_RAY_CACHE
+#define TSDF_RAY_CACHE 0  // raycast: 1 keeps a per-workgroup LDS cache of hash-lookup results
+#endif
 
 // One integrate (+1) or deintegrate (-1) operation.
 struct OpDesc {
@@ -1298,6 +1301,12 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     // the wavefront's start on the 100 MHz constant clock (its duration
     // orders the next raycast's tiles)
     const unsigned long long t_begin = __builtin_amdgcn_s_memrealtime();
+#if TSDF_RAY_CACHE
+    __shared__ unsigned long long rcache[kRayCache];
+    for (int e = threadIdx.x; e < kRayCache; e += blockDim.x)
+        rcache[e] = (unsigned long long)(uint32_t)(e ^ 1) << 32;
+    __syncthreads();
+#endif
     const int tiles_x = (rp.W + kRayTileW - 1) / kRayTileW;
     const int n_tiles = tiles_x * ((rp.H + kRayTileH - 1) / kRayTileH);
     // Workgroups are dealt to the 8 XCDs round-robin; with xcd_map the
@@ -1319,6 +1328,9 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     // empty-space certificate for this raycast (identical in every lane):
     // probed blocks lie within `reach` blocks of the camera's block
     RayParams rq = rp;
+#if TSDF_RAY_CACHE
+    rq.lc = kStats ? nullptr : rcache;
+#endif
     if (kStats) {
         rq.use_sb = 0;  // the footprint must see every probed key
         rq.use_tb = 0;
@@ -1939,6 +1951,7 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
     rp.use_tb = g->tune.raycast_skip == 2 && !kStats;
     // the group march replays no superblock distance field
     rp.budget = (kStats || rp.use_sb) ? 0 : g->tune.raycast_budget;
+    rp.lc = nullptr;  // the kernel points it at its LDS cache (TSDF_RAY_CACHE)
     const int qpar = (int)(g->ray_qepoch & 1u);
     if (rp.budget > 0)
         ++g->ray_qepoch;

@@ -283,7 +283,48 @@ struct RayParams {
     int df_lo[3], df_dim[3];
     int use_tb;       // tile depth bounds replay allowed (alias certificate holds)
     int budget;       // rounds of the per-lane march before a ray is deferred (0: never)
+    unsigned long long* lc;  // the workgroup's LDS cache of lookup results, or null
 };
+
+// The LDS lookup cache of a raycast workgroup (a measurement build,
+// TSDF_RAY_CACHE): kRayCache entries key << 32 | result, result = the brick
+// id of the key or 0xFFFFFFFF where the key is not in the map.  Entry e starts
+// as key e ^ 1, whose entry is not e, so it answers no lookup.  Entries are
+// written with 64-bit LDS atomics: two keys of one entry never mix.
+constexpr int kRayCache = 1024;
+
+// The first slot word of the lookup of `key`: from the cache where it holds
+// the key (a word first_slot reads as the cached result), else from the
+// table.
+__host__ __device__ __forceinline__ unsigned long long lookup_slot(const GridView& g, const RayParams& rp,
+                                                                 uint32_t key, uint32_t bk, bool& cached)
+{
+    cached = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (rp.lc) {
+        const unsigned long long e = rp.lc[key & (kRayCache - 1)];
+        if ((uint32_t)(e >> 32) == key) {
+            cached = true;
+            const uint32_t r = (uint32_t)e;
+            return r == 0xFFFFFFFFu ? kSlotEmpty : (((unsigned long long)key << 32) | r);
+        }
+    }
+#endif
+    return g.slots[bk];
+}
+
+// Records the result of a finished lookup (id >= 0, or -1) in the cache.
+__host__ __device__ __forceinline__ void cache_result(const RayParams& rp, uint32_t key, int id)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (rp.lc)
+        atomicExch(&rp.lc[key & (kRayCache - 1)], ((unsigned long long)key << 32) | (uint32_t)id);
+#else
+    (void)rp;
+    (void)key;
+    (void)id;
+#endif
+}
 
 // ---------------------------------------------------------------------------
 // Tile depth bounds: the raycast's exact empty-space skip.
@@ -667,6 +708,7 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
             fp_key(fp, pre_key);
         ++probes;
         id = r0 == kProbeMore ? probe_from(g, pre_key, (pre_bk + 1) & g.mask) : r0;
+        cache_result(rp, pre_key, id);
         first = false;
         if (id >= 0)
             return id;
@@ -710,6 +752,7 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
         // positions: rv itself first (on the first round), then steps
         uint32_t key[kSkipLook], bk[kSkipLook];
         unsigned long long sl[kSkipLook];
+        bool cached[kSkipLook];
         int dbl = 0, ev = 0;
         f3 r = rv;
 #pragma unroll
@@ -720,7 +763,7 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
             vb3(r, rp, x, y, z);
             key[k] = prime_hash(x, y, z);
             bk[k] = mix32(key[k]) & g.mask;
-            sl[k] = g.slots[bk[k]];
+            sl[k] = lookup_slot(g, rp, key[k], bk[k], cached[k]);
             if (k == kSkipLook - 1 && rp.use_sb) {
                 int idx;
                 dbl = df_box(rp, x, y, z, idx);
@@ -733,6 +776,8 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
 #pragma unroll
         for (int k = 0; k < kSkipLook; ++k) {
             const int r0 = first_slot(g, key[k], sl[k]);
+            if (!cached[k] && r0 != kProbeMore)
+                cache_result(rp, key[k], r0);
             if (kStats && go)
                 fp_key(fp, key[k]);
             const bool stp = go && !(first && k == 0);  // a step precedes this lookup
@@ -753,8 +798,10 @@ __host__ __device__ __forceinline__ int probe_run(const GridView& g, const RayPa
             nrep = df_dist(dbl, ev) - 4;
             have = true;
         }
-        if (pend)  // first slot held another key: finish the linear probe
+        if (pend) {  // first slot held another key: finish the linear probe
             id = probe_from(g, pkey, (pbk + 1) & g.mask);
+            cache_result(rp, pkey, id);
+        }
     }
     return id;
 }
@@ -971,7 +1018,8 @@ __host__ __device__ RayResult march(const GridView& g, const Pose& pose, const R
             if (e_found) {
                 e_key = prime_hash(ex, ey, ez);
                 e_bk = mix32(e_key) & g.mask;
-                e_slot = g.slots[e_bk];
+                bool e_cached;
+                e_slot = lookup_slot(g, rp, e_key, e_bk, e_cached);
             }
             bool stop = false;
             int ending = 0;  // 1 left the block / range, 2 UB index, 3 Fn, 4 crossing
