@@ -51,9 +51,6 @@ namespace {
 #ifndef TSDF_DEPTH_TILE
 #define TSDF_DEPTH_TILE 0  // plain update: 1 stages each brick's point-map rectangle in LDS
 #endif
-#ifndef TSDF_RAY_CACHE
-#define TSDF_RAY_CACHE 0  // raycast: 1 keeps a per-workgroup LDS cache of hash-lookup results
-#endif
 
 // One integrate (+1) or deintegrate (-1) operation.
 struct OpDesc {

@@ -292,6 +292,9 @@ struct RayParams {
 // as key e ^ 1, whose entry is not e, so it answers no lookup.  Entries are
 // written with 64-bit LDS atomics: two keys of one entry never mix.
 constexpr int kRayCache = 1024;
+#ifndef TSDF_RAY_CACHE
+#define TSDF_RAY_CACHE 0  // 1: k_raycast keeps the cache (the default build reads none)
+#endif
 
 // The first slot word of the lookup of `key`: from the cache where it holds
 // the key (a word first_slot reads as the cached result), else from the
@@ -300,7 +303,7 @@ __host__ __device__ __forceinline__ unsigned long long lookup_slot(const GridVie
                                                                  uint32_t key, uint32_t bk, bool& cached)
 {
     cached = false;
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(__HIP_DEVICE_COMPILE__) && TSDF_RAY_CACHE
     if (rp.lc) {
         const unsigned long long e = rp.lc[key & (kRayCache - 1)];
         if ((uint32_t)(e >> 32) == key) {
@@ -316,7 +319,7 @@ __host__ __device__ __forceinline__ unsigned long long lookup_slot(const GridVie
 // Records the result of a finished lookup (id >= 0, or -1) in the cache.
 __host__ __device__ __forceinline__ void cache_result(const RayParams& rp, uint32_t key, int id)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(__HIP_DEVICE_COMPILE__) && TSDF_RAY_CACHE
     if (rp.lc)
         atomicExch(&rp.lc[key & (kRayCache - 1)], ((unsigned long long)key << 32) | (uint32_t)id);
 #else
