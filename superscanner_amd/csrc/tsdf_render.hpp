@@ -415,16 +415,36 @@ __global__ __launch_bounds__(kThreads) void k_reduce_parts(const float* part, in
 
 // grad[key] += the values of key's run in the sorted records, summed in fp64
 // in record order (pixel order, the sort being stable); one thread per run.
+// The records of no voxel sort last, so a thread stops at the first one; a
+// run is read four records per round trip.
 __global__ __launch_bounds__(kThreads) void k_segment_add(const uint32_t* key, const float* val, size_t n,
                                                           float* grad)
 {
     for (size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x; p < n; p += (size_t)gridDim.x * kThreads) {
         const uint32_t k = key[p];
-        if (k == kNoRecord || (p > 0 && key[p - 1] == k))
+        if (k == kNoRecord)
+            break;
+        if (p > 0 && key[p - 1] == k)
             continue;
         double t = 0.0;
-        for (size_t q = p; q < n && key[q] == k; ++q)
-            t += (double)val[q];
+        for (size_t q = p; q < n; q += 4) {
+            uint32_t kk[4];
+            float vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool in = q + u < n;
+                kk[u] = in ? key[q + u] : kNoRecord;
+                vv[u] = in ? val[q + u] : 0.f;
+            }
+            bool more = true;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                more = more && kk[u] == k;
+                t = more ? t + (double)vv[u] : t;
+            }
+            if (!more)
+                break;
+        }
         grad[k] = (float)((double)grad[k] + t);
     }
 }
