@@ -357,7 +357,10 @@ __host__ __device__ __forceinline__ void cache_result(const RayParams& rp, uint3
 // atomicMin, so this raycast's entries are below every older one and a tile
 // without one reads as empty (every lookup of its rays misses).
 // ---------------------------------------------------------------------------
-constexpr int kTbTile = 8;  // tile side in pixels: the 8 x 8 pixels of a raycast wavefront
+#ifndef TSDF_TB_TILE
+#define TSDF_TB_TILE 8
+#endif
+constexpr int kTbTile = TSDF_TB_TILE;  // tile side in pixels: the 8 x 8 pixels of a raycast wavefront
 
 struct TileBounds {
     unsigned long long* z;  // [tiles_y * tiles_x]
