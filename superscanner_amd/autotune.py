@@ -99,10 +99,11 @@ def tune_integrate(grid, depths, poses, *, wg_per_cu=INTEGRATE_WG_PER_CU, rounds
 
 def tune_raycast(grid, poses, *, xcd_map=(0,), skip=(0, 1, 2), order=(0, 2), budget=(0, 12, 16), rounds=3):
     """Picks raycast_xcd_map, raycast_skip, raycast_order and raycast_budget
-    for ``grid.raycast``.  The candidates are timed in `rounds` interleaved
-    rounds and each keeps its best time.  Returns (best tuning dict,
-    [(xcd_map, skip, order, budget, ms), ...])."""
-    cands = list(itertools.product(xcd_map, skip, order, budget))
+    for ``grid.raycast`` (the group march replays no superblock field, so
+    skip 1 runs without a budget).  The candidates are timed in `rounds`
+    interleaved rounds and each keeps its best time.  Returns (best tuning
+    dict, [(xcd_map, skip, order, budget, ms), ...])."""
+    cands = [c for c in itertools.product(xcd_map, skip, order, budget) if not (c[1] == 1 and c[3])]
     best = {c: float("inf") for c in cands}
 
     def run():

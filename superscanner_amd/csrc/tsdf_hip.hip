@@ -1252,46 +1252,6 @@ __device__ __forceinline__ void write_ray(const RayResult& r, size_t o, const Po
 #endif
 constexpr int kRayGroup = TSDF_RAY_GROUP;
 
-// The ray parameters a raycast kernel marches with: the empty-space replays
-// only under this raycast's alias certificate (probed blocks lie within
-// `reach` blocks of the camera's block), and the distance field of the
-// current superblock bits.  k_raycast and k_raycast_tail derive the same.
-template <bool kStats>
-__device__ __forceinline__ RayParams kernel_ray_params(const GridView& g, const Pose& pose, const RayParams& rp)
-{
-    RayParams rq = rp;
-    if (kStats) {
-        rq.use_sb = 0;  // the footprint must see every probed key
-        rq.use_tb = 0;
-    } else if (rq.use_sb || rq.use_tb) {
-        const int reach = (int)(rp.dmax / rp.vbl) + 3;
-        const float* Ep = pose.E;
-        int c[3] = {vb_index(Ep[3], rp.vbl), vb_index(Ep[7], rp.vbl), vb_index(Ep[11], rp.vbl)};
-        int plo[3], phi[3], bmin[3], bmax[3];
-        for (int a = 0; a < 3; ++a) {
-            plo[a] = c[a] - reach;
-            phi[a] = c[a] + reach;
-            bmin[a] = g.ctrl->bbox_min[a];
-            bmax[a] = g.ctrl->bbox_max[a];
-        }
-        const int cert = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
-        rq.use_sb = rq.use_sb && cert;
-        rq.use_tb = rq.use_tb && cert;
-        // and a distance field of the current superblock bits
-        const SbdfInfo inf = rq.use_sb ? *g.sbdf_info : SbdfInfo{};
-        if (rq.use_sb && inf.valid && inf.version == g.ctrl->sb_version) {
-            rq.df = g.sbdf;
-            for (int a = 0; a < 3; ++a) {
-                rq.df_lo[a] = inf.lo[a];
-                rq.df_dim[a] = inf.dim[a];
-            }
-        } else {
-            rq.use_sb = 0;
-        }
-    }
-    return rq;
-}
-
 template <typename T, bool kColor>
 __global__ __launch_bounds__(kThreads) void k_raycast_tail(GridView g, Pose pose, RayParams rp, float* points,
                                                            float* colors, float* depth, uint8_t* ub,
@@ -1300,13 +1260,12 @@ __global__ __launch_bounds__(kThreads) void k_raycast_tail(GridView g, Pose pose
     if (blockIdx.x == 0 && threadIdx.x == 0)
         g.ctrl->ray_q[qpar ^ 1] = 0u;
     const Group<kRayGroup> gr;
-    const RayParams rq = kernel_ray_params<false>(g, pose, rp);
     const unsigned int n = g.ctrl->ray_q[qpar];
     const unsigned int groups = gridDim.x * (kThreads / kRayGroup);
     unsigned int hit = 0, bad = 0, samples = 0, probes = 0;
     for (unsigned int e = blockIdx.x * (kThreads / kRayGroup) + threadIdx.x / kRayGroup; e < n; e += groups) {
         const RayState st = queue[e];
-        const RayResult r = march_group<T, kColor, kRayGroup>(g, pose, rq, st, gr);
+        const RayResult r = march_group<T, kColor, kRayGroup>(g, pose, rp, st, gr);
         if (gr.t == 0) {
             write_ray(r, st.pix, pose, points, colors, depth, ub);
             hit += r.code == 1;
@@ -1363,10 +1322,41 @@ void k_raycast(GridView g, Pose pose, RayParams rp, float* points, float* colors
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = (tile / tiles_x) * kRayTileH + (wv / kRayWx) * 8 + (lane >> 3);
     const int j = (tile % tiles_x) * kRayTileW + (wv % kRayWx) * 8 + (lane & 7);
-    RayParams rq = kernel_ray_params<kStats>(g, pose, rp);
+    // empty-space certificate for this raycast (identical in every lane):
+    // probed blocks lie within `reach` blocks of the camera's block
+    RayParams rq = rp;
 #if TSDF_RAY_CACHE
     rq.lc = kStats ? nullptr : rcache;
 #endif
+    if (kStats) {
+        rq.use_sb = 0;  // the footprint must see every probed key
+        rq.use_tb = 0;
+    } else if (rq.use_sb || rq.use_tb) {
+        const int reach = (int)(rp.dmax / rp.vbl) + 3;
+        const float* Ep = pose.E;
+        int c[3] = {vb_index(Ep[3], rp.vbl), vb_index(Ep[7], rp.vbl), vb_index(Ep[11], rp.vbl)};
+        int plo[3], phi[3], bmin[3], bmax[3];
+        for (int a = 0; a < 3; ++a) {
+            plo[a] = c[a] - reach;
+            phi[a] = c[a] + reach;
+            bmin[a] = g.ctrl->bbox_min[a];
+            bmax[a] = g.ctrl->bbox_max[a];
+        }
+        const int cert = alias_free(bmin, bmax, plo, phi) ? 1 : 0;
+        rq.use_sb = rq.use_sb && cert;
+        rq.use_tb = rq.use_tb && cert;
+        // and a distance field of the current superblock bits
+        const SbdfInfo inf = rq.use_sb ? *g.sbdf_info : SbdfInfo{};
+        if (rq.use_sb && inf.valid && inf.version == g.ctrl->sb_version) {
+            rq.df = g.sbdf;
+            for (int a = 0; a < 3; ++a) {
+                rq.df_lo[a] = inf.lo[a];
+                rq.df_dim[a] = inf.dim[a];
+            }
+        } else {
+            rq.use_sb = 0;
+        }
+    }
     unsigned int hit = 0, bad = 0, samples = 0, probes = 0;
     // the depth bound of this wavefront's 8 x 8 pixels (one tile of the map)
     float zt = -1.0f;
@@ -1956,7 +1946,8 @@ void launch_raycast(tsdf_grid* g, const Pose& P, float* points, float* colors, f
         rp.order = 0;  // no previous raycast to order by yet
     rp.use_sb = g->tune.raycast_skip == 1;
     rp.use_tb = g->tune.raycast_skip == 2 && !kStats;
-    rp.budget = kStats ? 0 : g->tune.raycast_budget;
+    // the group march replays no superblock distance field
+    rp.budget = (kStats || rp.use_sb) ? 0 : g->tune.raycast_budget;
     rp.lc = nullptr;  // the kernel points it at its LDS cache (TSDF_RAY_CACHE)
     const int qpar = (int)(g->ray_qepoch & 1u);
     if (rp.budget > 0)

@@ -84,22 +84,6 @@ __device__ int gprobe_run(const GridView& g, const RayParams& rp, const Group<G>
                 ++probes;
             }
         }
-        // the empty-space replay, as in probe_run: every block within
-        // Chebyshev distance d - 1 of rv's block is absent, so the lookups of
-        // steps 1 .. d - 4 miss (every lane of the group computes the same)
-        if (rp.use_sb && !first) {
-            int cx, cy, cz;
-            vb3(rv, rp, cx, cy, cz);
-            const int nrep = free_dist(rp, cx, cy, cz) - 4;
-            for (int k = 0; k < nrep && l < dmax; ++k) {
-                rv = add3(rv, delta);
-                l += dl;
-                ++steps;
-                ++probes;
-            }
-            if (!(l < dmax))
-                break;
-        }
         const int nst = first ? gr.t : gr.t + 1;
         f3 r = rv;
         float lr = l;
