@@ -12,7 +12,9 @@ against the oracle:
   * fp16 storage at 1280x720 with 2^24 buckets on the multi-room scene
     against the fp64 oracle on a crop (error bounds stated in the test);
   * a batch of 300 ops (more than one staging chunk of 256) against the
-    plain calls; the fp16 weight cap; colour refused by a grid without it.
+    plain calls; the fp16 weight cap; colour refused by a grid without it;
+  * the timed workload's 600 frames integrated and deintegrated again: every
+    voxel back to zero (a size-independent property at the full size).
 """
 import numpy as np
 import pytest
@@ -263,3 +265,57 @@ def test_colour_refused_without_colour_storage(gpu):
     assert g.size() > 0
     g.close()
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("storage,color", [("f32", False), ("f32", True), ("f16", False)])
+def test_room_600_frames_integrate_deintegrate_round_trip(gpu, storage, color):
+    """The timed workload's 600 frames (848x480, 2^22 buckets) integrated,
+    then deintegrated in reverse order: every voxel returns to w = 0 and
+    sdf = 0 (and colour 0) exactly, whatever the float rounding of the
+    running averages in between -- inside its band a call moves the weight by
+    exactly 1, and the deintegrate at w = 1 sets weight, sdf and colour to 0
+    (GPUVoxelGrid.cpp:367-395, 426-469) -- and the raycast then finds no
+    surface.  A size-independent property at the full size; weights uncapped
+    (fp32) or capped above the 600 calls (fp16, exact integers to 2048)."""
+    cfg = synth.ROOM_LARGE
+    n = 600
+    H, W = cfg["H"], cfg["W"]
+    poses64 = synth.traj_random_walk(n=n, seed=2)
+    depth = synth.frames(synth.scene_room(seed=1), poses64, cfg, seed=7)
+    poses = [synth.pose_f32(T) for T in poses64]
+    g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
+                     n_bricks=1 << 21, withColor=color, max_weight=0.0 if storage == "f32" else 2048.0,
+                     storage=storage)
+    g.setIntrinsic(*cfg["intr"])
+    dd = torch.from_numpy(depth.view(np.int16)).cuda()
+    del depth
+
+    def rgb(k):
+        if not color:
+            return None
+        return dev_u8(np.random.default_rng(1000 + k).integers(0, 256, size=(H, W, 3), dtype=np.uint8))
+
+    for k in range(n):
+        g.integrate(rgb(k), dd[k], poses[k])
+    g.sync()
+    mid = g.export_bricks()
+    n_bricks = mid["w"].shape[0]
+    assert n_bricks > 20000 and float(mid["w"].max()) > 100.0  # voxels seen by many frames
+    del mid
+    for k in reversed(range(n)):
+        g.deIntegrate(rgb(k), dd[k], poses[k])
+    g.sync()
+    assert g.stats()["overflow"] == 0
+    ex = g.export_bricks()
+    assert ex["w"].shape[0] == n_bricks
+    assert int((ex["w"] != 0).sum()) == 0, "weights left after the round trip"
+    sdf_bits = ex["sdf"].float().contiguous().view(torch.int32)
+    assert int((sdf_bits != 0).sum()) == 0, "sdf left after the round trip (not +0.0)"
+    if color:
+        assert int((ex["rgb"] != 0).sum()) == 0, "colour left after the round trip"
+    for k in (0, 299, 599):
+        g.raycast(poses[k], fence=True)
+        ub = g.rayUB.cpu().numpy()
+        dep = g.rayDepth.cpu().numpy()
+        assert int((dep[ub == 0] != 0).sum()) == 0, "a surface found in an all-zero map"
+    g.close()
