@@ -4,7 +4,8 @@ against the oracle:
   * room schedule (848x480, vl 4 mm, trunc 20 mm, 2^22 buckets, max weight
     128): the timed sequence's own seeds and call order -- integrate frame i,
     deintegrate frame i - 10 at every 10th pose, raycast -- over its first 70
-    poses; grid, counters and raycasts bit-identical;
+    poses, and over all 600 (where weights reach the cap); grid, counters and
+    raycasts bit-identical;
   * batched reintegration of 64 keyframes (848x480, valid fraction U(0.3, 1)
     per keyframe, poses perturbed by N(1 cm, 0.5 deg)): one
     tsdf_reintegrate_batch (one captured graph) against the oracle's 128
@@ -65,6 +66,46 @@ def test_room_schedule_first_70_poses(gpu):
     assert s["ray_probes"] - before["ray_probes"] == probes
     assert_grids_equal(g, o)
     g.close()
+
+
+def test_room_schedule_all_600_poses(gpu):
+    """The whole timed step (600 poses: integrate frame i, deintegrate frame
+    i - 10 at every 10th pose) against the oracle, with raycasts at three
+    poses: over 600 frames many voxels reach the weight cap of 128, so the
+    cap's rule runs at scale.  Grid, counters and raycasts bit-identical."""
+    import os
+    cfg = synth.ROOM_LARGE
+    n = 600
+    poses64 = synth.traj_random_walk(n=n, seed=2)
+    depth = synth.frames(synth.scene_room(seed=1), poses64, cfg, seed=7)
+    poses = [synth.pose_f32(T) for T in poses64]
+    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], cfg["intr"],
+                     max_weight=cfg["max_weight"], n_buckets=cfg["n_buckets"], n_bricks=1 << 21)
+    env = os.environ.get("OMP_NUM_THREADS")
+    oracle.set_threads(int(env) if env else len(os.sched_getaffinity(0)))
+    try:
+        tot = dict(n_touched=0, n_new=0, n_upd=0)
+        for i in range(n):
+            ops = [(i, 1)] + ([(i - 10, -1)] if i >= 10 and i % 10 == 0 else [])
+            for k, s in ops:
+                (g.integrate if s > 0 else g.deIntegrate)(None, dev_u16(depth[k]), poses[k])
+                st, c = o.integrate(depth[k], poses[k], s)
+                assert st == 0
+                for key in tot:
+                    tot[key] += c[key]
+            if i in (299, 450, n - 1):
+                r = assert_raycast_equal(g, o, poses[i])
+                assert r["counts"]["n_hit"] > 0.5 * cfg["H"] * cfg["W"]
+        g.sync()
+        s = g.stats()
+        assert s["overflow"] == 0 and s["n_bricks"] == o.n_bricks()
+        assert s["total_touched"] == tot["n_touched"] and s["total_new"] == tot["n_new"]
+        assert s["total_updated"] == tot["n_upd"]
+        eg, _ = assert_grids_equal(g, o)
+        assert float(eg["w"].max()) == cfg["max_weight"], "the cap was not reached"
+    finally:
+        oracle.set_threads(1)
+        g.close()
 
 
 def keyframe_workload(keyframes=64):
