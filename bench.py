@@ -594,6 +594,33 @@ def fp16_scale_leg(dev, frames):
         out[st] = {"ms": round(ms, 1), "ms_per_frame": round(ms / frames, 3),
                    "Mpx_per_s": round(frames * H * W / ms / 1e3, 1), "bricks": stats["n_bricks"],
                    "payload_GB": round(stats["n_bricks"] * 512 * 2 * bpv / 1e9, 2)}
+        if st == "f16":
+            # the same sequence again (untimed above) with an event pair
+            # around every launch: per-kernel times, and the update's
+            # algorithmic bytes (touched bricks' payload read, updated
+            # voxels written, slots) at 4 B per fp16 voxel
+            g.profile(True)
+            g.profile_read(reset=True)
+            run.step()
+            torch.cuda.synchronize()
+            kt = g.profile_read(reset=True)
+            g.profile(False)
+            g.sync()
+            s2 = g.stats()
+            n_int = frames
+            touched = s2["total_touched"] / n_int
+            upd = s2["total_updated"] / n_int
+            kern = {}
+            for name in ("collect", "integrate", "raycast"):
+                k = kt[name]
+                if k["launches"]:
+                    kern[name] = {"avg_us": round(k["ms"] * 1e3 / k["launches"], 2)}
+            ib = touched * 512 * 2 * bpv + upd * 2 * bpv + touched * S_H
+            if "integrate" in kern:
+                gbs = ib / (kern["integrate"]["avg_us"] * 1e-6) / 1e9
+                kern["integrate"].update({"bytes_per_launch": int(ib), "achieved_GBs": round(gbs, 1),
+                                          "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            out[st]["kernels"] = kern
         e = g.export_bricks()
         c = e["coords"].long()
         key = ((c[:, 0] + (1 << 20)) << 42) | ((c[:, 1] + (1 << 20)) << 21) | (c[:, 2] + (1 << 20))
