@@ -68,17 +68,27 @@ def test_room_schedule_first_70_poses(gpu):
     g.close()
 
 
+_ROOM_600 = {}
+
+
+def room_600():
+    """The timed workload's 600 frames (generated once per module)."""
+    if not _ROOM_600:
+        cfg = synth.ROOM_LARGE
+        poses64 = synth.traj_random_walk(n=600, seed=2)
+        _ROOM_600["depth"] = synth.frames(synth.scene_room(seed=1), poses64, cfg, seed=7)
+        _ROOM_600["poses"] = [synth.pose_f32(T) for T in poses64]
+    return synth.ROOM_LARGE, _ROOM_600["poses"], _ROOM_600["depth"]
+
+
 def test_room_schedule_all_600_poses(gpu):
     """The whole timed step (600 poses: integrate frame i, deintegrate frame
     i - 10 at every 10th pose) against the oracle, with raycasts at three
     poses: over 600 frames many voxels reach the weight cap of 128, so the
     cap's rule runs at scale.  Grid, counters and raycasts bit-identical."""
     import os
-    cfg = synth.ROOM_LARGE
+    cfg, poses, depth = room_600()
     n = 600
-    poses64 = synth.traj_random_walk(n=n, seed=2)
-    depth = synth.frames(synth.scene_room(seed=1), poses64, cfg, seed=7)
-    poses = [synth.pose_f32(T) for T in poses64]
     g, o = make_pair(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], cfg["intr"],
                      max_weight=cfg["max_weight"], n_buckets=cfg["n_buckets"], n_bricks=1 << 21)
     env = os.environ.get("OMP_NUM_THREADS")
@@ -318,18 +328,14 @@ def test_room_600_frames_integrate_deintegrate_round_trip(gpu, storage, color):
     (GPUVoxelGrid.cpp:367-395, 426-469) -- and the raycast then finds no
     surface.  A size-independent property at the full size; weights uncapped
     (fp32) or capped above the 600 calls (fp16, exact integers to 2048)."""
-    cfg = synth.ROOM_LARGE
+    cfg, poses, depth = room_600()
     n = 600
     H, W = cfg["H"], cfg["W"]
-    poses64 = synth.traj_random_walk(n=n, seed=2)
-    depth = synth.frames(synth.scene_room(seed=1), poses64, cfg, seed=7)
-    poses = [synth.pose_f32(T) for T in poses64]
     g = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
                      n_bricks=1 << 21, withColor=color, max_weight=0.0 if storage == "f32" else 2048.0,
                      storage=storage)
     g.setIntrinsic(*cfg["intr"])
     dd = torch.from_numpy(depth.view(np.int16)).cuda()
-    del depth
 
     def rgb(k):
         if not color:
