@@ -337,10 +337,13 @@ def test_room_600_frames_integrate_deintegrate_round_trip(gpu, storage, color):
     g.setIntrinsic(*cfg["intr"])
     dd = torch.from_numpy(depth.view(np.int16)).cuda()
 
+    colours = None
+    if color:  # every frame's colour image, drawn once on the device
+        gen = torch.Generator(device="cuda").manual_seed(1000)
+        colours = torch.randint(0, 256, (n, H, W, 3), dtype=torch.uint8, device="cuda", generator=gen)
+
     def rgb(k):
-        if not color:
-            return None
-        return dev_u8(np.random.default_rng(1000 + k).integers(0, 256, size=(H, W, 3), dtype=np.uint8))
+        return colours[k] if color else None
 
     for k in range(n):
         g.integrate(rgb(k), dd[k], poses[k])
