@@ -1,11 +1,21 @@
 """Shared helpers of the GPU parity tests: run the HIP path through the C ABI
 (via superscanner_amd.voxel_grid) and the oracle on the same inputs, and
 compare the results bit for bit."""
+import os
+
 import numpy as np
 import torch
 
 import oracle
 from superscanner_amd.voxel_grid import GPUVoxelGrid
+
+
+def oracle_threads() -> int:
+    """The host cores this process is allotted (OMP_NUM_THREADS where set):
+    the oracle's threaded block collection gives the serial loop's result
+    (tests/test_oracle_parallel.py), so the parity tests use them all."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return int(env) if env and env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
 
 
 def dev_u16(a):
@@ -21,6 +31,7 @@ def make_pair(vl, trunc, H, W, intr, *, with_color=False, max_weight=0.0, n_buck
     g = GPUVoxelGrid(n_bricks * 512, trunc, vl, H, W, n_buckets=n_buckets, n_bricks=n_bricks,
                      withColor=with_color, max_weight=max_weight, storage=storage)
     g.setIntrinsic(*intr)
+    oracle.set_threads(oracle_threads())
     o = oracle.OracleGrid(vl, trunc, H, W, max_bricks=n_bricks, with_color=with_color,
                           max_weight=max_weight)
     o.set_intrinsics(*intr)

@@ -86,13 +86,10 @@ def test_room_schedule_all_600_poses(gpu):
     i - 10 at every 10th pose) against the oracle, with raycasts at three
     poses: over 600 frames many voxels reach the weight cap of 128, so the
     cap's rule runs at scale.  Grid, counters and raycasts bit-identical."""
-    import os
     cfg, poses, depth = room_600()
     n = 600
     g, o = make_pair(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], cfg["intr"],
                      max_weight=cfg["max_weight"], n_buckets=cfg["n_buckets"], n_bricks=1 << 21)
-    env = os.environ.get("OMP_NUM_THREADS")
-    oracle.set_threads(int(env) if env else len(os.sched_getaffinity(0)))
     try:
         tot = dict(n_touched=0, n_new=0, n_upd=0)
         for i in range(n):
@@ -114,7 +111,6 @@ def test_room_schedule_all_600_poses(gpu):
         eg, _ = assert_grids_equal(g, o)
         assert float(eg["w"].max()) == cfg["max_weight"], "the cap was not reached"
     finally:
-        oracle.set_threads(1)
         g.close()
 
 
