@@ -81,34 +81,43 @@ def room_600():
     return synth.ROOM_LARGE, _ROOM_600["poses"], _ROOM_600["depth"]
 
 
-def test_room_schedule_all_600_poses(gpu):
+@pytest.mark.parametrize("color", [False, True])
+def test_room_schedule_all_600_poses(gpu, color):
     """The whole timed step (600 poses: integrate frame i, deintegrate frame
     i - 10 at every 10th pose) against the oracle, with raycasts at three
     poses: over 600 frames many voxels reach the weight cap of 128, so the
-    cap's rule runs at scale.  Grid, counters and raycasts bit-identical."""
+    cap's rule runs at scale.  Grid, counters and raycasts bit-identical;
+    with colour (the reference's default, GPUVoxelGrid.h:99) the running
+    colour averages and the raycast colours too."""
     cfg, poses, depth = room_600()
     n = 600
-    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], cfg["intr"],
-                     max_weight=cfg["max_weight"], n_buckets=cfg["n_buckets"], n_bricks=1 << 21)
+    H, W = cfg["H"], cfg["W"]
+    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], H, W, cfg["intr"], with_color=color,
+                     max_weight=cfg["max_weight"], n_buckets=cfg["n_buckets"], n_bricks=1 << 17)
+    rgbs = None
+    if color:  # one colour image per frame, on the host for the oracle
+        rgbs = np.random.default_rng(17).integers(0, 256, size=(n, H, W, 3), dtype=np.uint8)
     try:
         tot = dict(n_touched=0, n_new=0, n_upd=0)
         for i in range(n):
             ops = [(i, 1)] + ([(i - 10, -1)] if i >= 10 and i % 10 == 0 else [])
             for k, s in ops:
-                (g.integrate if s > 0 else g.deIntegrate)(None, dev_u16(depth[k]), poses[k])
-                st, c = o.integrate(depth[k], poses[k], s)
+                rgb = rgbs[k] if color else None
+                (g.integrate if s > 0 else g.deIntegrate)(dev_u8(rgb) if color else None, dev_u16(depth[k]),
+                                                          poses[k])
+                st, c = o.integrate(depth[k], poses[k], s, rgb)
                 assert st == 0
                 for key in tot:
                     tot[key] += c[key]
             if i in (299, 450, n - 1):
-                r = assert_raycast_equal(g, o, poses[i])
+                r = assert_raycast_equal(g, o, poses[i], color=color)
                 assert r["counts"]["n_hit"] > 0.5 * cfg["H"] * cfg["W"]
         g.sync()
         s = g.stats()
         assert s["overflow"] == 0 and s["n_bricks"] == o.n_bricks()
         assert s["total_touched"] == tot["n_touched"] and s["total_new"] == tot["n_new"]
         assert s["total_updated"] == tot["n_upd"]
-        eg, _ = assert_grids_equal(g, o)
+        eg, _ = assert_grids_equal(g, o, color=color)
         assert float(eg["w"].max()) == cfg["max_weight"], "the cap was not reached"
     finally:
         g.close()
