@@ -137,29 +137,33 @@ def keyframe_workload(keyframes=64):
     return cfg, kdepth, old, new, keep
 
 
-def test_batched_reintegration_64_keyframes(gpu):
+@pytest.mark.parametrize("color", [False, True])
+def test_batched_reintegration_64_keyframes(gpu, color):
     cfg, kdepth, old, new, keep = keyframe_workload(64)
     valid = (kdepth > 0).mean(axis=(1, 2))
     assert valid.min() < 0.4 and valid.max() > 0.8  # uneven valid-pixel counts
-    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], cfg["H"], cfg["W"], cfg["intr"],
+    H, W = cfg["H"], cfg["W"]
+    g, o = make_pair(cfg["voxel_length"], cfg["truncation"], H, W, cfg["intr"], with_color=color,
                      max_weight=cfg["max_weight"], n_buckets=cfg["n_buckets"], n_bricks=1 << 18)
     kd = [dev_u16(d) for d in kdepth]
+    rgb = np.random.default_rng(23).integers(0, 256, size=(len(kd), H, W, 3), dtype=np.uint8) if color else None
+    kc = [dev_u8(c) for c in rgb] if color else None
     for k in range(len(kd)):
-        g.integrate(None, kd[k], old[k])
-        o.integrate(kdepth[k], old[k])
+        g.integrate(kc[k] if color else None, kd[k], old[k])
+        o.integrate(kdepth[k], old[k], 1, rgb[k] if color else None)
     g.sync()
     s0 = g.stats()
-    g.reintegrate_batch(kd, old, new)
+    g.reintegrate_batch(kd, old, new, rgbs=kc)
     for k in range(len(kd)):  # integrate.cpp:108-125: deIntegrate(old) then integrate(new)
-        o.integrate(kdepth[k], old[k], -1)
-        o.integrate(kdepth[k], new[k], 1)
+        o.integrate(kdepth[k], old[k], -1, rgb[k] if color else None)
+        o.integrate(kdepth[k], new[k], 1, rgb[k] if color else None)
     g.sync()
     s1 = g.stats()
     assert s1["overflow"] == 0 and s1["n_bricks"] == o.n_bricks()
     assert s1["total_updated"] > s0["total_updated"]
-    assert_grids_equal(g, o)
+    assert_grids_equal(g, o, color=color)
     for k in (0, 31, 63):
-        assert_raycast_equal(g, o, new[k])
+        assert_raycast_equal(g, o, new[k], color=color)
     g.close()
 
 
