@@ -4,12 +4,13 @@ against the oracle:
   * room schedule (848x480, vl 4 mm, trunc 20 mm, 2^22 buckets, max weight
     128): the timed sequence's own seeds and call order -- integrate frame i,
     deintegrate frame i - 10 at every 10th pose, raycast -- over its first 70
-    poses, and over all 600 (where weights reach the cap); grid, counters and
-    raycasts bit-identical;
+    poses, and over all 600 (where weights reach the cap), with and without
+    colour; grid, counters and raycasts bit-identical;
   * batched reintegration of 64 keyframes (848x480, valid fraction U(0.3, 1)
     per keyframe, poses perturbed by N(1 cm, 0.5 deg)): one
     tsdf_reintegrate_batch (one captured graph) against the oracle's 128
-    plain calls in the order of integrate.cpp:108-125; bit-identical;
+    plain calls in the order of integrate.cpp:108-125, with and without
+    colour; bit-identical;
   * fp16 storage at 1280x720 with 2^24 buckets on the multi-room scene
     against the fp64 oracle on a crop (error bounds stated in the test);
   * a batch of 300 ops (more than one staging chunk of 256) against the
