@@ -70,7 +70,7 @@ struct Offsets {
 
 constexpr int kThreads = 256;
 #ifndef TSDF_TILE_W
-#define TSDF_TILE_W 32
+#define TSDF_TILE_W 64
 #endif
 #ifndef TSDF_TILE_H
 #define TSDF_TILE_H 8
@@ -78,10 +78,11 @@ constexpr int kThreads = 256;
 #ifndef TSDF_LDS_SET
 #define TSDF_LDS_SET 1024
 #endif
-// Collection tile: 32 x 8 pixels = 4 patches of 8 x 8, one per wavefront.
-// Its block set holds kLdsSet blocks (8 KiB); a tile sees about 40 blocks on
-// average, but grazing views put hundreds in one tile, and a full set falls
-// back to claiming blocks one by one, which costs far more than the set.
+// Collection tile: 64 x 8 pixels = 8 patches of 8 x 8, two per wavefront.
+// Its block set holds kLdsSet blocks (8 KiB); a tile sees several tens of
+// blocks on average, but grazing views put hundreds in one tile, and a full
+// set falls back to claiming blocks one by one, which costs far more than
+// the set.
 constexpr int kTileW = TSDF_TILE_W;
 constexpr int kTileH = TSDF_TILE_H;
 constexpr int kLdsSet = TSDF_LDS_SET;
@@ -328,7 +329,7 @@ __global__ void k_reset(GridView g, uint32_t* op_mask, size_t op_mask_words)
 // ---------------------------------------------------------------------------
 // k_collect: block allocation (vbCollectionTag, GPUVoxelGrid.cpp:471-514)
 // ---------------------------------------------------------------------------
-// One workgroup per 32 x 8 pixel tile.  Each valid pixel enumerates the
+// One workgroup per kTileW x kTileH pixel tile.  Each valid pixel enumerates the
 // blocks of its +-trunc box into an LDS set, so the global table sees each
 // block once per tile; then the tile's set is claimed against the global
 // table with wavefront-aggregated appends to the touched list.  The pixel's
@@ -346,7 +347,7 @@ __device__ void collect_tile(const GridView& g, const OpDesc& op, uint32_t epoch
         s_valid = 0;
     __syncthreads();
 
-    // Each wavefront takes 8 x 8 pixel patches of the 32 x 8 tile (lane =
+    // Each wavefront takes 8 x 8 pixel patches of the tile (lane =
     // pixel of the patch).  The blocks of a patch are the union of its
     // pixels' +-trunc boxes; when their common bounding box has at most 128
     // cells, that union is built exactly as a 128-bit mask over the box (each
