@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--fp16-frames", type=int, default=2000)
     ap.add_argument("--footprint-samples", type=int, default=12)
+    ap.add_argument("--tuning", type=str, default=None, metavar="JSON",
+                    help="launch shapes to use instead of the autotuner's (a tsdf_tuning dict, e.g. for a "
+                         "profile of the steps alone); the raycast order is not re-settled")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run 2 steps of the workload plus a 256 MiB calibration copy and exit "
                          "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
@@ -980,9 +983,14 @@ def main():
     grid = GPUVoxelGrid(1 << 30, cfg["truncation"], cfg["voxel_length"], H, W, n_buckets=cfg["n_buckets"],
                         n_bricks=1 << 21, max_weight=cfg["max_weight"])
     grid.setIntrinsic(*cfg["intr"])
-    tuning = autotune_plain(cfg, depth_dev, poses, frames)
-    grid.set_tuning(**tuning)
-    print(f"[bench] autotuned launch shapes: {tuning}", file=sys.stderr, flush=True)
+    if args.tuning:
+        grid.set_tuning(**json.loads(args.tuning))
+        tuning = grid.get_tuning()
+        print(f"[bench] given launch shapes: {tuning}", file=sys.stderr, flush=True)
+    else:
+        tuning = autotune_plain(cfg, depth_dev, poses, frames)
+        grid.set_tuning(**tuning)
+        print(f"[bench] autotuned launch shapes: {tuning}", file=sys.stderr, flush=True)
     run = Runner(grid, depth_dev, poses, frames)
 
     for _ in range(args.warmup):
@@ -992,7 +1000,7 @@ def main():
     # tiles that slowest-first dispatch puts first, and splits between the
     # two orders, while over the whole sequence slowest-first is ~13 % faster.
     order_ms = {}
-    for od in (2, 0):
+    for od in ((2, 0) if not args.tuning else ()):
         grid.set_tuning(raycast_order=od)
         run.step()  # builds the tile costs the order is taken from
         oa, ob = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -1001,9 +1009,10 @@ def main():
         ob.record()
         ob.synchronize()
         order_ms[od] = round(oa.elapsed_time(ob), 2)
-    tuning["raycast_order"] = min(order_ms, key=order_ms.get)
-    grid.set_tuning(raycast_order=tuning["raycast_order"])
-    run.step()
+    if order_ms:
+        tuning["raycast_order"] = min(order_ms, key=order_ms.get)
+        grid.set_tuning(raycast_order=tuning["raycast_order"])
+        run.step()
     torch.cuda.synchronize()
     print(f"[bench] raycast order on the timed workload: step ms {order_ms}", file=sys.stderr, flush=True)
     st = grid.stats()  # counters of one step (each step starts from reset)
