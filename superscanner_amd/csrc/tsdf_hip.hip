@@ -91,8 +91,11 @@ constexpr int kLdsSet = TSDF_LDS_SET;
 #ifndef TSDF_BATCH_TILE_W
 #define TSDF_BATCH_TILE_W 64
 #endif
+#ifndef TSDF_BATCH_TILE_H
+#define TSDF_BATCH_TILE_H 16
+#endif
 constexpr int kBatchTileW = TSDF_BATCH_TILE_W;
-constexpr int kBatchTileH = 16;
+constexpr int kBatchTileH = TSDF_BATCH_TILE_H;
 constexpr int kBatchLdsSet = 2048;  // four times the plain tile's pixels
 constexpr int kMaxOps = 256;    // ops per batched pass
 constexpr int kMaskWords = kMaxOps / 32;
